@@ -1,0 +1,65 @@
+/*
+ * zpic_gpu.h - additions of libzpic_em2d to the em2d C API.
+ *
+ * None of these functions exists in the reference.  The em2d entry points
+ * (sim_iter, spec_advance, current_update, emf_advance, ...) keep their
+ * reference signatures and run on the GPU; these calls manage the host
+ * mirrors of the device-resident state and expose tuning and test hooks.
+ *
+ * Host mirrors: after a step the particle buffer `spec->part`/`spec->np`,
+ * `spec->energy` and the grids `E_buf`, `B_buf`, `J_buf` are stale until
+ * the matching *_sync_host call.  A caller that edits a mirror calls the
+ * matching *_mark_host_dirty so the edit is uploaded before the next step.
+ * The reference's Python module handed out views of these buffers
+ * (python/source/em2d.pyx:305-312, 1044-1150, 1567-1619); its replacement
+ * calls these functions around every access.
+ */
+#ifndef ZPIC_GPU_H
+#define ZPIC_GPU_H
+
+#include <stdint.h>
+
+#include "simulation.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Select the HIP device before the first GPU call (default: current device). */
+void zpic_set_device(int device);
+
+/* Tile shape for species/currents created afterwards: tx x ty cells with a
+ * drift margin of `halo` cells; 0 keeps the default (16, 16, 2).  Tiles
+ * must divide the grid; a side that no preferred tile width divides uses
+ * the largest divisor <= 64 that leaves room for the margin. */
+void zpic_set_tiling(int tx, int ty, int halo);
+
+/* Block until all queued GPU work is done. */
+void zpic_synchronize(void);
+
+/* Refresh host mirrors from the device. */
+void spec_sync_host(t_species *spec);
+void emf_sync_host(t_emf *emf);
+void current_sync_host(t_current *current);
+void sim_sync_host(t_simulation *sim);
+
+/* Declare a host mirror edited: it is uploaded before the next step. */
+void spec_mark_host_dirty(t_species *spec);
+void emf_mark_host_dirty(t_emf *emf);
+
+/* Particle identities for one-to-one comparisons: after spec_track_ids the
+ * particles present get ids 0..np-1 in host-buffer order and injected ones
+ * continue the count.  spec_get_ids copies the ids in the order of the
+ * host mirror filled by the last spec_sync_host. */
+void spec_track_ids(t_species *spec);
+void spec_get_ids(t_species *spec, int64_t *ids);
+
+/* Number of pushes of the last step that left their tile window (took the
+ * global-memory path); a diagnostic of the tile margin. */
+int spec_overflow_count(t_species *spec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

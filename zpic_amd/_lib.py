@@ -1,0 +1,47 @@
+"""Loader of the HIP library libzpic_em2d.so (built by __graft_entry__.build()).
+
+There is no CPU fallback: importing the em2d API without the compiled
+library raises, so every caller runs the GPU kernels or fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import abi as A
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzpic_em2d.so")
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """The em2d GPU library with argument types declared."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libzpic_em2d.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    A.bind_api(lib, A.make_abi(True))
+    P = C.POINTER
+    ext = {
+        "zpic_set_device": (None, [C.c_int]),
+        "zpic_set_tiling": (None, [C.c_int, C.c_int, C.c_int]),
+        "zpic_synchronize": (None, []),
+        "spec_sync_host": (None, [C.c_void_p]),
+        "emf_sync_host": (None, [C.c_void_p]),
+        "current_sync_host": (None, [C.c_void_p]),
+        "sim_sync_host": (None, [C.c_void_p]),
+        "spec_mark_host_dirty": (None, [C.c_void_p]),
+        "emf_mark_host_dirty": (None, [C.c_void_p]),
+        "spec_track_ids": (None, [C.c_void_p]),
+        "spec_get_ids": (None, [C.c_void_p, P(C.c_int64)]),
+        "spec_overflow_count": (C.c_int, [C.c_void_p]),
+    }
+    for name, (res, args) in ext.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

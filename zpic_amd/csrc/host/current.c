@@ -1,0 +1,164 @@
+/*
+ * current.c - electric current density on the GPU.
+ *
+ * Host side of the reference em2d/current.c.  current_zero opens a step
+ * (the next deposit overwrites the per-tile windows); current_update sums
+ * the windows into J with the periodic fold of the reference's guard cells
+ * and applies the digital filter passes (em2d/current.c:171-183, 427-459).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "zpic_internal.h"
+
+void current_new(t_current *c, int nx[], float box[], float dt)
+{
+    /* em2d/current.c:30-79 */
+    for (int d = 0; d < 2; d++) {
+        c->nx[d] = nx[d];
+        c->gc[d][0] = 1;
+        c->gc[d][1] = 2;
+        c->box[d] = box[d];
+        c->dx[d] = box[d] / nx[d];
+    }
+    c->nrow = nx[0] + 3;
+    c->J_buf = calloc(zpic_grid_nodes(c->nx), sizeof(float3));
+    if (!c->J_buf) zpic_fatal("out of host memory for the current grid");
+    c->J = c->J_buf + 1 + c->nrow;
+    c->smooth.xtype = NONE;
+    c->smooth.ytype = NONE;
+    c->smooth.xlevel = 0;
+    c->smooth.ylevel = 0;
+    c->iter = 0;
+    c->dt = dt;
+    c->moving_window = 0;
+    c->dev = calloc(1, sizeof(struct zpic_dev_current));
+    c->dev->host_valid = 1;
+    c->dev->fresh = 1;
+}
+
+void current_delete(t_current *c)
+{
+    struct zpic_dev_current *d = c->dev;
+    if (d) {
+        zpic_sync();
+        zpic_dfree(d->J[0]);
+        zpic_dfree(d->J[1]);
+        zpic_dfree(d->region);
+        zpic_dfree(d->Jov);
+        free(d);
+        c->dev = NULL;
+    }
+    free(c->J_buf);
+    c->J_buf = NULL;
+}
+
+void zpic_current_ensure(t_current *c, const zk_tiles *t)
+{
+    struct zpic_dev_current *d = c->dev;
+    if (!d) zpic_fatal("current density was not created with current_new");
+    if (!d->ready) {
+        zk_tiles tt;
+        if (t) tt = *t;
+        else zpic_tiles_for(c->nx[0], c->nx[1], &tt);
+        d->t = tt;
+        const size_t nodes = zpic_grid_nodes(c->nx);
+        d->J[0] = zpic_dmalloc(sizeof(float3) * nodes);
+        d->J[1] = zpic_dmalloc(sizeof(float3) * nodes);
+        d->Jov = zpic_dmalloc(sizeof(float3) * nodes);
+        d->region = zpic_dmalloc(sizeof(float) * 3 * (size_t)tt.WX * (size_t)tt.WY * (size_t)tt.ntiles);
+        ZCHECK(zk_fill_f32(d->J[0], 3 * nodes, 0.0f, zpic_stream()));
+        ZCHECK(zk_fill_f32(d->Jov, 3 * nodes, 0.0f, zpic_stream()));
+        d->jcur = 0;
+        d->ready = 1;
+    } else if (t && (t->TX != d->t.TX || t->TY != d->t.TY || t->H != d->t.H || t->nx != d->t.nx ||
+                     t->ny != d->t.ny)) {
+        zpic_fatal("species and current density use different grids or tilings");
+    }
+}
+
+void current_zero(t_current *c)
+{
+    /* em2d/current.c:98-107: every deposit of the step starts from zero */
+    zpic_current_ensure(c, NULL);
+    struct zpic_dev_current *d = c->dev;
+    ZCHECK(zk_fill_f32(d->Jov, 3 * zpic_grid_nodes(c->nx), 0.0f, zpic_stream()));
+    d->fresh = 1;
+    d->host_valid = 0;
+}
+
+/* Weights of the compensating pass after n binomial passes
+ * (em2d/current.c:297-304). */
+static void compensator(int n, float *sa, float *sb)
+{
+    float a = -1;
+    float b = (4.0 + 2.0 * n) / n;
+    float total = 2 * a + b;
+    *sa = a / total;
+    *sb = b / total;
+}
+
+static void smooth_pass(t_current *c, int axis, float sa, float sb)
+{
+    struct zpic_dev_current *d = c->dev;
+    ZCHECK(zk_smooth(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, axis, sa, sb,
+                     !c->moving_window, zpic_stream()));
+    d->jcur = 1 - d->jcur;
+}
+
+void current_update(t_current *c)
+{
+    zpic_current_ensure(c, NULL);
+    struct zpic_dev_current *d = c->dev;
+    zk_reduce_args a;
+    a.region = d->fresh ? NULL : d->region;
+    a.Jov = d->Jov;
+    a.J = d->J[d->jcur];
+    a.nrow = c->nrow;
+    a.t = d->t;
+    a.periodic_x = !c->moving_window;
+    ZCHECK(zk_current_reduce(&a, zpic_stream()));
+
+    /* binomial passes, then the compensator; the y pass count is xlevel,
+     * as in the reference (em2d/current.c:449) */
+    float sa, sb;
+    if (c->smooth.xtype != NONE) {
+        for (int k = 0; k < c->smooth.xlevel; k++) smooth_pass(c, 0, 0.25f, 0.5f);
+        if (c->smooth.xtype == COMPENSATED) {
+            compensator(c->smooth.xlevel, &sa, &sb);
+            smooth_pass(c, 0, sa, sb);
+        }
+    }
+    if (c->smooth.ytype != NONE) {
+        for (int k = 0; k < c->smooth.xlevel; k++) smooth_pass(c, 1, 0.25f, 0.5f);
+        if (c->smooth.ytype == COMPENSATED) {
+            compensator(c->smooth.ylevel, &sa, &sb);
+            smooth_pass(c, 1, sa, sb);
+        }
+    }
+    c->iter++;
+    d->host_valid = 0;
+}
+
+void current_sync_host(t_current *c)
+{
+    struct zpic_dev_current *d = c->dev;
+    if (!d || !d->ready || d->host_valid) return;
+    zpic_d2h(c->J_buf, d->J[d->jcur], sizeof(float3) * zpic_grid_nodes(c->nx));
+    d->host_valid = 1;
+}
+
+const float *zpic_current_J(t_current *c) { return c->dev->J[c->dev->jcur]; }
+
+void current_report(const t_current *c, const int jc)
+{
+    static int warned = 0;
+    (void)c;
+    if (jc < 0 || jc > 2) {
+        fprintf(stderr, "(*error*) Invalid current component (jc) selected, returning\n");
+        return;
+    }
+    if (!warned) fprintf(stderr, "zpic_em2d: ZDF diagnostic files are not written by this library\n");
+    warned = 1;
+}

@@ -1,0 +1,721 @@
+/*
+ * particles.c - particle species of the em2d time step on the GPU.
+ *
+ * Host side of the reference em2d/particles.c: species construction and
+ * particle injection run on the host with the reference's arithmetic and
+ * random-number order (so initial states and moving-window injections are
+ * bit-identical to the reference); the push, deposit, boundary conditions
+ * and sort of spec_advance run as HIP kernels on particles kept in device
+ * memory, grouped by tile.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "random.h"
+#include "timer.h"
+#include "zpic_internal.h"
+
+/* all live species, for the process-wide push counter */
+static struct zpic_dev_species *g_live = NULL;
+static unsigned long long g_npush_deleted = 0;
+
+/* ------------------------------------------------------------------------
+ * injection on the host (em2d/particles.c:85-492)
+ * ---------------------------------------------------------------------- */
+static float unit_density(float x, void *data)
+{
+    (void)x;
+    (void)data;
+    return 1.0f;
+}
+
+/* Upper bound of the particles a cell range receives (em2d/particles.c:367-449);
+ * the slab's upper edge divides by dx[1], as the reference does. */
+static int inject_bound(const t_species *s, const int r[2][2])
+{
+    const int rows = r[1][1] - r[1][0] + 1;
+    switch (s->density.type) {
+    case STEP: {
+        int i0 = s->density.start / s->dx[0] - s->n_move;
+        if (i0 > r[0][1]) return 0;
+        if (i0 < r[0][0]) i0 = r[0][0];
+        return (r[0][1] - i0 + 1) * s->ppc[0] * rows * s->ppc[1];
+    }
+    case SLAB: {
+        int i0 = s->density.start / s->dx[0] - s->n_move;
+        int i1 = s->density.end / s->dx[1] - s->n_move;
+        if (i0 > r[0][1] || i1 < r[0][0]) return 0;
+        if (i0 < r[0][0]) i0 = r[0][0];
+        if (i1 > r[0][1]) i1 = r[0][1];
+        return (i1 - i0 + 1) * s->ppc[0] * rows * s->ppc[1];
+    }
+    case CUSTOM: {
+        const t_density *d = &s->density;
+        /* trapezoid sums: end points halved, then the inner points */
+        double qx = (*d->custom_x)((r[0][0] + s->n_move) * s->dx[0], d->custom_data_x);
+        qx += (*d->custom_x)((r[0][1] + 1 + s->n_move) * s->dx[0], d->custom_data_x);
+        qx *= 0.5;
+        for (int i = r[0][0] + 1; i <= r[0][1]; i++) qx += (*d->custom_x)((i + s->n_move) * s->dx[0], d->custom_data_x);
+        double qy = (*d->custom_y)(r[1][0] * s->dx[1], d->custom_data_y);
+        qy += (*d->custom_y)((r[1][1] + 1) * s->dx[1], d->custom_data_y);
+        qy *= 0.5;
+        for (int j = r[1][0] + 1; j <= r[1][1]; j++) qy += (*d->custom_y)(j * s->dx[1], d->custom_data_y);
+        return ceil(qx * s->ppc[0]) * ceil(qy * s->ppc[1]);
+    }
+    case EMPTY:
+        return 0;
+    default:
+        return (r[0][1] - r[0][0] + 1) * s->ppc[0] * rows * s->ppc[1];
+    }
+}
+
+/* Lattice positions of a cell range for the density profile; particles are
+ * written at out[0..), the count is returned (em2d/particles.c:159-354). */
+static int inject_positions(t_species *s, const int r[2][2], t_part *out)
+{
+    const int npc = s->ppc[0] * s->ppc[1];
+    const float hx = 1.0f / s->ppc[0], hy = 1.0f / s->ppc[1];
+    float *lat = malloc(sizeof(float) * 2 * (size_t)(npc > 0 ? npc : 1));
+    for (int b = 0, k = 0; b < s->ppc[1]; b++)
+        for (int a = 0; a < s->ppc[0]; a++, k++) {
+            lat[2 * k] = hx * (a + 0.5);
+            lat[2 * k + 1] = hy * (b + 0.5);
+        }
+    int n = 0;
+#define EMIT(I, J, X, Y)                                                                                               \
+    do {                                                                                                               \
+        out[n].ix = (I);                                                                                               \
+        out[n].iy = (J);                                                                                               \
+        out[n].x = (X);                                                                                                \
+        out[n].y = (Y);                                                                                                \
+        n++;                                                                                                           \
+    } while (0)
+    if (s->density.type == STEP || s->density.type == SLAB) {
+        const float lo = s->density.start / s->dx[0] - s->n_move;
+        const float hi = s->density.end / s->dx[0] - s->n_move;
+        const int slab = (s->density.type == SLAB);
+        for (int j = r[1][0]; j <= r[1][1]; j++)
+            for (int i = r[0][0]; i <= r[0][1]; i++)
+                for (int k = 0; k < npc; k++)
+                    if (i + lat[2 * k] > lo && (!slab || i + lat[2 * k] < hi)) EMIT(i, j, lat[2 * k], lat[2 * k + 1]);
+    } else if (s->density.type == CUSTOM) {
+        /* invert the trapezoidal cumulative density along x, continuing
+         * from earlier injections, and along y from the bottom */
+        t_density *d = &s->density;
+        const double hxd = s->dx[0], hyd = s->dx[1];
+        const double px = 1.0 / s->ppc[0], py = 1.0 / s->ppc[1];
+        const double thr = 4 * px * py;
+        unsigned long kx = d->custom_x_total_part;
+        double cx1 = d->custom_x_total_q;
+        double fx1 = (*d->custom_x)((r[0][0] + s->n_move) * hxd, d->custom_data_x);
+        const double fy_first = (*d->custom_y)(r[1][0] * hyd, d->custom_data_y);
+        for (int i = r[0][0]; i <= r[0][1]; i++) {
+            const double fx0 = fx1;
+            fx1 = (*d->custom_x)((i + 1 + s->n_move) * hxd, d->custom_data_x);
+            const double cx0 = cx1;
+            cx1 += 0.5 * (fx0 + fx1);
+            double tx;
+            while ((tx = (kx + 0.5) * px) < cx1) {
+                const double xp = 2 * (tx - cx0) / (sqrt(fx0 * fx0 + 2 * (fx1 - fx0) * (tx - cx0)) + fx0);
+                const double wxd = (0.5 - xp) * fx0 + (0.5 + xp) * fx1;
+                unsigned long ky = 0;
+                double fy1 = fy_first, cy1 = 0;
+                for (int j = r[1][0]; j <= r[1][1]; j++) {
+                    const double fy0 = fy1;
+                    fy1 = (*d->custom_y)((j + 1) * hyd, d->custom_data_y);
+                    const double cy0 = cy1;
+                    cy1 += 0.5 * (fy0 + fy1);
+                    double ty;
+                    while ((ty = (ky + 0.5) * py) < cy1) {
+                        const double yp = 2 * (ty - cy0) / (sqrt(fy0 * fy0 + 2 * (fy1 - fy0) * (ty - cy0)) + fy0);
+                        const double wyd = (0.5 - yp) * fy0 + (0.5 + yp) * fy1;
+                        if (wxd * wyd > thr) EMIT(i, j, (float)xp, (float)yp);
+                        ky++;
+                    }
+                }
+                kx++;
+            }
+        }
+        d->custom_x_total_q = cx1;
+        d->custom_x_total_part = kx;
+    } else if (s->density.type != EMPTY) {
+        for (int j = r[1][0]; j <= r[1][1]; j++)
+            for (int i = r[0][0]; i <= r[0][1]; i++)
+                for (int k = 0; k < npc; k++) EMIT(i, j, lat[2 * k], lat[2 * k + 1]);
+    }
+#undef EMIT
+    free(lat);
+    return n;
+}
+
+/* Thermal momenta with the per-cell mean removed, plus the fluid drift
+ * (em2d/particles.c:85-149).  The cell key uses nx[1] as the row stride,
+ * as the reference does. */
+static void inject_momenta(const t_species *s, t_part *p, int n)
+{
+    for (int i = 0; i < n; i++) {
+        p[i].ux = s->uth[0] * rand_norm();
+        p[i].uy = s->uth[1] * rand_norm();
+        p[i].uz = s->uth[2] * rand_norm();
+    }
+    const int cells = s->nx[0] * s->nx[1];
+    float3 *mean = calloc((size_t)cells, sizeof(float3));
+    int *count = calloc((size_t)cells, sizeof(int));
+    for (int i = 0; i < n; i++) {
+        const int c = p[i].ix + s->nx[1] * p[i].iy;
+        mean[c].x += p[i].ux;
+        mean[c].y += p[i].uy;
+        mean[c].z += p[i].uz;
+        count[c] += 1;
+    }
+    for (int c = 0; c < cells; c++) {
+        const float w = (count[c] > 0) ? 1.0f / count[c] : 0;
+        mean[c].x *= w;
+        mean[c].y *= w;
+        mean[c].z *= w;
+    }
+    for (int i = 0; i < n; i++) {
+        const int c = p[i].ix + s->nx[1] * p[i].iy;
+        p[i].ux += s->ufl[0] - mean[c].x;
+        p[i].uy += s->ufl[1] - mean[c].y;
+        p[i].uz += s->ufl[2] - mean[c].z;
+    }
+    free(count);
+    free(mean);
+}
+
+void spec_grow_buffer(t_species *s, const int size)
+{
+    if (size > s->np_max) {
+        s->np_max = (size / 1024 + 1) * 1024;
+        s->part = realloc(s->part, sizeof(t_part) * (size_t)s->np_max);
+        if (!s->part) zpic_fatal("out of host memory for %d particles", s->np_max);
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * construction
+ * ---------------------------------------------------------------------- */
+void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const float ufl[], const float uth[],
+              const int nx[], float box[], const float dt, t_density *density)
+{
+    /* em2d/particles.c:535-609 */
+    strncpy(s->name, name, MAX_SPNAME_LEN);
+    s->name[MAX_SPNAME_LEN] = 0;
+    int npc = 1;
+    for (int d = 0; d < 2; d++) {
+        s->nx[d] = nx[d];
+        s->ppc[d] = ppc[d];
+        npc *= ppc[d];
+        s->box[d] = box[d];
+        s->dx[d] = box[d] / nx[d];
+    }
+    s->m_q = m_q;
+    s->q = copysign(1.0f, m_q) / npc;
+    s->dt = dt;
+    s->np_max = 0;
+    s->part = NULL;
+    if (density) {
+        s->density = *density;
+    } else {
+        memset(&s->density, 0, sizeof(s->density));
+        s->density.type = UNIFORM;
+        s->density.n = 1.0;
+    }
+    if (s->density.n == 0.) s->density.n = 1.0;
+    if (s->density.type == CUSTOM) {
+        if (!s->density.custom_x) s->density.custom_x = unit_density;
+        if (!s->density.custom_y) s->density.custom_y = unit_density;
+        s->density.custom_x_total_part = 0;
+        s->density.custom_x_total_q = 0;
+    }
+    s->q *= fabsf(s->density.n);
+    for (int d = 0; d < 3; d++) {
+        s->ufl[d] = ufl ? ufl[d] : 0;
+        s->uth[d] = uth ? uth[d] : 0;
+    }
+    s->iter = 0;
+    s->moving_window = 0;
+    s->n_move = 0;
+    s->energy = 0;
+    s->np = 0;
+
+    const int r[2][2] = {{0, nx[0] - 1}, {0, nx[1] - 1}};
+    spec_grow_buffer(s, inject_bound(s, r));
+    s->np = inject_positions(s, r, s->part);
+    inject_momenta(s, s->part, s->np);
+    s->n_sort = 16;
+
+    struct zpic_dev_species *d = calloc(1, sizeof(*d));
+    d->host_dirty = 1;
+    d->host_valid = 1;
+    d->owner = s;
+    d->next_live = g_live;
+    g_live = d;
+    s->dev = d;
+}
+
+static void species_free_device(struct zpic_dev_species *d)
+{
+    if (d->ready) {
+        zk_spec_ctr c;
+        zpic_d2h(&c, d->ctr, sizeof(c));
+        d->npush_retired += c.npush;
+    }
+    for (int k = 0; k < 2; k++) {
+        zk_parts *p = &d->set[k];
+        zpic_dfree(p->ix);
+        zpic_dfree(p->iy);
+        zpic_dfree(p->x);
+        zpic_dfree(p->y);
+        zpic_dfree(p->ux);
+        zpic_dfree(p->uy);
+        zpic_dfree(p->uz);
+        zpic_dfree(p->id);
+        memset(p, 0, sizeof(*p));
+    }
+    zpic_dfree(d->tile_off);
+    zpic_dfree(d->cnt);
+    zpic_dfree(d->cursor);
+    zpic_dfree(d->epart);
+    zpic_dfree(d->stage);
+    zpic_dfree(d->ctr);
+    d->tile_off = d->cnt = d->cursor = NULL;
+    d->epart = NULL;
+    d->stage = NULL;
+    d->ctr = NULL;
+    d->ready = 0;
+}
+
+static unsigned long long device_npush(struct zpic_dev_species *d)
+{
+    if (!d->ready) return d->npush_retired;
+    zk_spec_ctr c;
+    zpic_d2h(&c, d->ctr, sizeof(c));
+    return d->npush_retired + c.npush;
+}
+
+void spec_delete(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (d) {
+        zpic_sync();
+        species_free_device(d);
+        g_npush_deleted += d->npush_retired;
+        struct zpic_dev_species **pp = &g_live;
+        while (*pp && *pp != d) pp = &(*pp)->next_live;
+        if (*pp) *pp = d->next_live;
+        free(d->host_ids);
+        free(d);
+        s->dev = NULL;
+    }
+    free(s->part);
+    s->part = NULL;
+    s->np = -1;
+}
+
+uint64_t spec_npush(void)
+{
+    unsigned long long n = g_npush_deleted;
+    for (struct zpic_dev_species *d = g_live; d; d = d->next_live) n += device_npush(d);
+    return n;
+}
+
+double spec_time(void) { return zpic_timer_total(ZPIC_T_SPEC); }
+
+double spec_perf(void)
+{
+    const uint64_t n = spec_npush();
+    return n > 0 ? spec_time() / (double)n : -1.0;
+}
+
+/* ------------------------------------------------------------------------
+ * device residency
+ * ---------------------------------------------------------------------- */
+static void alloc_set(zk_parts *p, int cap, int ids)
+{
+    p->ix = zpic_dmalloc(sizeof(int) * (size_t)cap);
+    p->iy = zpic_dmalloc(sizeof(int) * (size_t)cap);
+    p->x = zpic_dmalloc(sizeof(float) * (size_t)cap);
+    p->y = zpic_dmalloc(sizeof(float) * (size_t)cap);
+    p->ux = zpic_dmalloc(sizeof(float) * (size_t)cap);
+    p->uy = zpic_dmalloc(sizeof(float) * (size_t)cap);
+    p->uz = zpic_dmalloc(sizeof(float) * (size_t)cap);
+    p->id = ids ? zpic_dmalloc(sizeof(int64_t) * (size_t)cap) : NULL;
+}
+
+static void ensure_stage(struct zpic_dev_species *d, size_t bytes)
+{
+    if (bytes <= d->stage_cap) return;
+    zpic_sync();
+    zpic_dfree(d->stage);
+    d->stage = zpic_dmalloc(bytes);
+    d->stage_cap = bytes;
+}
+
+/* Grow both buffer sets to hold at least `need` particles. */
+static void grow_sets(struct zpic_dev_species *d, int need)
+{
+    if (need <= d->cap) return;
+    int cap = need + need / 8 + 4096;
+    zpic_sync();
+    for (int k = 0; k < 2; k++) {
+        zk_parts old = d->set[k], nw;
+        alloc_set(&nw, cap, d->track_ids);
+        if (k == d->cur && old.ix && d->np_upper > 0) {
+            const size_t n = (size_t)d->np_upper;
+            zpic_d2d_async(nw.ix, old.ix, 4 * n);
+            zpic_d2d_async(nw.iy, old.iy, 4 * n);
+            zpic_d2d_async(nw.x, old.x, 4 * n);
+            zpic_d2d_async(nw.y, old.y, 4 * n);
+            zpic_d2d_async(nw.ux, old.ux, 4 * n);
+            zpic_d2d_async(nw.uy, old.uy, 4 * n);
+            zpic_d2d_async(nw.uz, old.uz, 4 * n);
+            if (old.id && nw.id)
+                zpic_d2d_async(nw.id, old.id, 8 * n);
+        }
+        zpic_sync();
+        zpic_dfree(old.ix);
+        zpic_dfree(old.iy);
+        zpic_dfree(old.x);
+        zpic_dfree(old.y);
+        zpic_dfree(old.ux);
+        zpic_dfree(old.uy);
+        zpic_dfree(old.uz);
+        zpic_dfree(old.id);
+        d->set[k] = nw;
+    }
+    d->cap = cap;
+}
+
+static void sort_now(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    const int other = 1 - d->cur;
+    ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, &d->t, d->np_upper,
+                   zpic_stream()));
+    d->cur = other;
+    d->tail_upper = 0;
+    d->moved_since_sort = 0;
+}
+
+/* Upload the host mirror into the device buffers and sort it into tiles. */
+static void upload(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    const int np = s->np > 0 ? s->np : 0;
+    /* the kernels index grids by cell: refuse particles outside the grid */
+    for (int i = 0; i < np; i++) {
+        const t_part *p = &s->part[i];
+        if (p->ix < 0 || p->ix >= s->nx[0] || p->iy < 0 || p->iy >= s->nx[1])
+            zpic_fatal("species '%s': particle %d has cell (%d, %d) outside the %d x %d grid", s->name, i, p->ix,
+                       p->iy, s->nx[0], s->nx[1]);
+    }
+    grow_sets(d, np);
+    ensure_stage(d, sizeof(t_part) * (size_t)(np > 0 ? np : 1));
+    zpic_h2d(d->stage, s->part, sizeof(t_part) * (size_t)np);
+    ZCHECK(zk_unpack(d->stage, np, &d->set[d->cur], 0, d->ctr, 0, zpic_stream()));
+    if (d->track_ids && np > 0) zpic_h2d(d->set[d->cur].id, d->host_ids, sizeof(int64_t) * (size_t)np);
+    d->np_upper = np;
+    d->tail_upper = np;
+    sort_now(s);
+    d->host_dirty = 0;
+}
+
+void zpic_species_ensure(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d) zpic_fatal("species '%s' was not created with spec_new", s->name);
+    if (!d->ready) {
+        zpic_tiles_for(s->nx[0], s->nx[1], &d->t);
+        int want = s->np > 0 ? s->np : 1;
+        if (s->moving_window) {
+            /* a full box of particles plus a few columns of injections */
+            long long full = (long long)s->nx[0] * s->nx[1] * s->ppc[0] * s->ppc[1];
+            if (full > want) want = (int)full;
+        }
+        d->cap = 0;
+        d->cur = 0;
+        grow_sets(d, want);
+        d->ctr = zpic_dmalloc(sizeof(zk_spec_ctr));
+        const zk_spec_ctr zero = {0};
+        zpic_h2d(d->ctr, &zero, sizeof(zero));
+        d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
+        d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->cursor = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        ZCHECK(zk_fill_f32((float *)d->cnt, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */
+        d->epart_cap = d->t.ntiles + 1024;
+        d->epart = zpic_dmalloc(sizeof(double) * (size_t)d->epart_cap);
+        d->ready = 1;
+        d->host_dirty = 1;
+    }
+    if (d->host_dirty) upload(s);
+}
+
+void spec_mark_host_dirty(t_species *s)
+{
+    if (s->dev) {
+        s->dev->host_dirty = 1;
+        s->dev->host_valid = 1;
+    }
+}
+
+void spec_sync_host(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d || !d->ready || d->host_valid || d->host_dirty) return;
+    zk_spec_ctr c;
+    zpic_d2h(&c, d->ctr, sizeof(c));
+    if (c.n_dead > 0 || c.n_sorted != c.np_total) {
+        sort_now(s);
+        zpic_d2h(&c, d->ctr, sizeof(c));
+    }
+    const int np = c.np_total;
+    d->np_upper = np;
+    spec_grow_buffer(s, np);
+    ensure_stage(d, sizeof(t_part) * (size_t)(np > 0 ? np : 1));
+    ZCHECK(zk_pack(&d->set[d->cur], np, d->stage, zpic_stream()));
+    zpic_d2h(s->part, d->stage, sizeof(t_part) * (size_t)np);
+    if (d->track_ids) {
+        if (d->host_ids_cap < s->np_max) {
+            d->host_ids = realloc(d->host_ids, sizeof(int64_t) * (size_t)s->np_max);
+            d->host_ids_cap = s->np_max;
+        }
+        zpic_d2h(d->host_ids, d->set[d->cur].id, sizeof(int64_t) * (size_t)np);
+    }
+    s->np = np;
+    s->energy = c.energy;
+    d->host_valid = 1;
+}
+
+void spec_track_ids(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d || d->track_ids) return;
+    spec_sync_host(s);
+    d->host_ids_cap = s->np_max > 0 ? s->np_max : 1;
+    d->host_ids = realloc(d->host_ids, sizeof(int64_t) * (size_t)d->host_ids_cap);
+    for (int i = 0; i < s->np; i++) d->host_ids[i] = i;
+    d->next_id = s->np;
+    d->track_ids = 1;
+    if (d->ready) {
+        /* re-create the buffers with id arrays */
+        species_free_device(d);
+    }
+    d->host_dirty = 1;
+}
+
+void spec_get_ids(t_species *s, int64_t *ids)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d || !d->track_ids) zpic_fatal("spec_get_ids: ids are not tracked for species '%s'", s->name);
+    spec_sync_host(s);
+    memcpy(ids, d->host_ids, sizeof(int64_t) * (size_t)s->np);
+}
+
+void zpic_spec_sync_energy(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d || !d->ready || d->host_valid) return;
+    zk_spec_ctr c;
+    zpic_d2h(&c, d->ctr, sizeof(c));
+    s->energy = c.energy;
+}
+
+int spec_overflow_count(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d || !d->ready) return 0;
+    zk_spec_ctr c;
+    zpic_d2h(&c, d->ctr, sizeof(c));
+    return c.n_overflow;
+}
+
+/* ------------------------------------------------------------------------
+ * time step (em2d/particles.c:1104-1269)
+ * ---------------------------------------------------------------------- */
+/* Inject the rightmost column after a window move (em2d/particles.c:619-640),
+ * appending to the unsorted tail on the device. */
+static void inject_column(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    const int r[2][2] = {{s->nx[0] - 1, s->nx[0] - 1}, {0, s->nx[1] - 1}};
+    const int bound = inject_bound(s, r);
+    t_part *tmp = malloc(sizeof(t_part) * (size_t)(bound > 0 ? bound : 1));
+    const int n = inject_positions(s, r, tmp);
+    inject_momenta(s, tmp, n);
+    if (n > 0) {
+        grow_sets(d, d->np_upper + n);
+        ensure_stage(d, sizeof(t_part) * (size_t)n);
+        zpic_h2d(d->stage, tmp, sizeof(t_part) * (size_t)n);
+        ZCHECK(zk_unpack(d->stage, n, &d->set[d->cur], d->next_id, d->ctr, 1, zpic_stream()));
+        d->next_id += n;
+        d->np_upper += n;
+        d->tail_upper += n;
+    }
+    free(tmp);
+}
+
+void spec_advance(t_species *s, t_emf *emf, t_current *cur)
+{
+    zpic_species_ensure(s);
+    struct zpic_dev_species *d = s->dev;
+    zpic_emf_ensure(emf);
+    zpic_current_ensure(cur, &d->t);
+    struct zpic_dev_current *dc = cur->dev;
+
+    zpic_timer_begin(ZPIC_T_SPEC);
+    zk_push_args a;
+    memset(&a, 0, sizeof(a));
+    a.p = d->set[d->cur];
+    a.tile_off = d->tile_off;
+    a.ctr = d->ctr;
+    a.E = zpic_emf_particle_E(emf);
+    a.B = zpic_emf_particle_B(emf);
+    a.nrow = emf->nrow;
+    a.region = dc->region;
+    a.fresh = dc->fresh;
+    a.Jov = dc->Jov;
+    a.epart = d->epart;
+    a.t = d->t;
+    a.moving_window = s->moving_window;
+    /* constants with the reference's precision (em2d/particles.c:1111-1117) */
+    a.tem = 0.5 * s->dt / s->m_q;
+    a.dt_dx = s->dt / s->dx[0];
+    a.dt_dy = s->dt / s->dx[1];
+    a.qnx = s->q * s->dx[0] / s->dt;
+    a.qny = s->q * s->dx[1] / s->dt;
+    a.q = s->q;
+    const int next_iter = s->iter + 1;
+    const int move = s->moving_window && ((next_iter * s->dt) > (s->dx[0] * (s->n_move + 1)));
+    a.shift = move;
+    int tb = d->tail_upper > 0 ? (d->tail_upper + 255) / 256 : 0;
+    if (tb > 1024) tb = 1024;
+    if (d->t.ntiles + tb > d->epart_cap) {
+        zpic_sync();
+        zpic_dfree(d->epart);
+        d->epart_cap = d->t.ntiles + tb + 1024;
+        d->epart = zpic_dmalloc(sizeof(double) * (size_t)d->epart_cap);
+        a.epart = d->epart;
+    }
+    a.tail_blocks = tb;
+    ZCHECK(zk_push(&a, zpic_stream()));
+    dc->fresh = 0;
+
+    s->iter = next_iter;
+    if (move) {
+        s->n_move++;
+        inject_column(s);
+        d->moved_since_sort++;
+    }
+    const float qm = s->q * s->m_q;
+    ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0], (double)s->dx[1],
+                            zpic_stream()));
+    if ((s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0) sort_now(s);
+    zpic_timer_end(ZPIC_T_SPEC);
+    d->host_valid = 0;
+}
+
+void spec_move_window(t_species *s)
+{
+    /* standalone form of the window move (em2d/particles.c:619-640), on the
+     * host mirror; spec_advance applies the shift inside the push kernel */
+    if ((s->iter * s->dt) > (s->dx[0] * (s->n_move + 1))) {
+        spec_sync_host(s);
+        for (int i = 0; i < s->np; i++) s->part[i].ix--;
+        s->n_move++;
+        const int r[2][2] = {{s->nx[0] - 1, s->nx[0] - 1}, {0, s->nx[1] - 1}};
+        spec_grow_buffer(s, s->np + inject_bound(s, r));
+        const int n = inject_positions(s, r, s->part + s->np);
+        inject_momenta(s, s->part + s->np, n);
+        if (s->dev && s->dev->track_ids) {
+            struct zpic_dev_species *d = s->dev;
+            if (d->host_ids_cap < s->np_max) {
+                d->host_ids = realloc(d->host_ids, sizeof(int64_t) * (size_t)s->np_max);
+                d->host_ids_cap = s->np_max;
+            }
+            for (int i = 0; i < n; i++) d->host_ids[s->np + i] = d->next_id++;
+        }
+        s->np += n;
+        spec_mark_host_dirty(s);
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * diagnostics
+ * ---------------------------------------------------------------------- */
+void spec_deposit_charge(const t_species *cs, float *charge)
+{
+    /* CIC deposit on the device, boundary fold on the host
+     * (em2d/particles.c:1289-1324) */
+    t_species *s = (t_species *)cs;
+    zpic_species_ensure(s);
+    struct zpic_dev_species *d = s->dev;
+    const int nrow = s->nx[0] + 1;
+    const size_t n = (size_t)nrow * (size_t)(s->nx[1] + 1);
+    float *rho = zpic_dmalloc(sizeof(float) * n);
+    ZCHECK(zk_fill_f32(rho, n, 0.0f, zpic_stream()));
+    ZCHECK(zk_charge(&d->set[d->cur], d->np_upper, d->ctr, s->q, s->nx[0], rho, zpic_stream()));
+    float *h = malloc(sizeof(float) * n);
+    zpic_d2h(h, rho, sizeof(float) * n);
+    zpic_dfree(rho);
+    for (size_t k = 0; k < n; k++) charge[k] += h[k];
+    free(h);
+    if (!s->moving_window)
+        for (int j = 0; j < s->nx[1] + 1; j++) charge[j * nrow] += charge[s->nx[0] + j * nrow];
+    for (int i = 0; i < s->nx[0] + 1; i++) charge[i] += charge[i + s->nx[1] * nrow];
+}
+
+static float pha_quantity(const t_species *s, int i, int q)
+{
+    switch (q) {
+    case X1: return (s->part[i].x + s->part[i].ix) * s->dx[0];
+    case X2: return (s->part[i].y + s->part[i].iy) * s->dx[1];
+    case U1: return s->part[i].ux;
+    case U2: return s->part[i].uy;
+    case U3: return s->part[i].uz;
+    }
+    return 0;
+}
+
+void spec_deposit_pha(const t_species *cs, const int rep_type, const int pha_nx[], const float pha_range[][2],
+                      float *buf)
+{
+    /* phase-space density from the host mirror (em2d/particles.c:1569-1632) */
+    t_species *s = (t_species *)cs;
+    spec_sync_host(s);
+    const int nrow = pha_nx[0];
+    const int qa = rep_type & 0x000F, qb = (rep_type & 0x00F0) >> 4;
+    const float amin = pha_range[0][0], bmin = pha_range[1][0];
+    const float ra = pha_nx[0] / (pha_range[0][1] - pha_range[0][0]);
+    const float rb = pha_nx[1] / (pha_range[1][1] - pha_range[1][0]);
+    for (int i = 0; i < s->np; i++) {
+        const float na = (pha_quantity(s, i, qa) - amin) * ra;
+        const float nb = (pha_quantity(s, i, qb) - bmin) * rb;
+        const int ia = (int)(na + 0.5f), ib = (int)(nb + 0.5f);
+        const float wa = na - ia + 0.5f, wb = nb - ib + 0.5f;
+        int c = ia + nrow * ib;
+        if (ib >= 0 && ib < pha_nx[1]) {
+            if (ia >= 0 && ia < pha_nx[0]) buf[c] += (1.0f - wa) * (1.0f - wb) * s->q;
+            if (ia + 1 >= 0 && ia + 1 < pha_nx[0]) buf[c + 1] += wa * (1.0f - wb) * s->q;
+        }
+        c += nrow;
+        if (ib + 1 >= 0 && ib + 1 < pha_nx[1]) {
+            if (ia >= 0 && ia < pha_nx[0]) buf[c] += (1.0f - wa) * wb * s->q;
+            if (ia + 1 >= 0 && ia + 1 < pha_nx[0]) buf[c + 1] += wa * wb * s->q;
+        }
+    }
+}
+
+void spec_report(const t_species *s, const int rep_type, const int pha_nx[], const float pha_range[][2])
+{
+    static int warned = 0;
+    (void)s;
+    (void)rep_type;
+    (void)pha_nx;
+    (void)pha_range;
+    if (!warned) fprintf(stderr, "zpic_em2d: ZDF diagnostic files are not written by this library\n");
+    warned = 1;
+}

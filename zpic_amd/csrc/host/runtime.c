@@ -1,0 +1,203 @@
+/*
+ * runtime.c - HIP device, stream, memory, timers and tile choice.
+ *
+ * The em2d API is single-threaded and not reentrant, like the reference
+ * (process-global timers and RNG, em2d/particles.c:26-27, em2d/emf.c:27,
+ * em2d/random.c:16-17), so one process-wide stream carries all work.
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "zpic_rt.h"
+
+static int g_device = -1;
+static int g_ready = 0;
+static hipStream_t g_stream;
+
+void zpic_set_device(int device);
+void zpic_set_tiling(int tx, int ty, int halo);
+void zpic_synchronize(void);
+
+static int g_tx = 16, g_ty = 16, g_halo = 2;
+
+void zpic_check(int status, const char *what, const char *file, int line)
+{
+    if (status != 0) {
+        fprintf(stderr, "zpic_em2d: HIP error '%s' in %s (%s:%d), aborting.\n",
+                hipGetErrorString((hipError_t)status), what, file, line);
+        exit(-1);
+    }
+}
+
+void zpic_fatal(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "zpic_em2d: ");
+    vfprintf(stderr, fmt, ap);
+    fprintf(stderr, "\n");
+    va_end(ap);
+    exit(-1);
+}
+
+void zpic_set_device(int device)
+{
+    if (g_ready && device != g_device) zpic_fatal("zpic_set_device must be called before the first GPU use");
+    g_device = device;
+}
+
+void zpic_set_tiling(int tx, int ty, int halo)
+{
+    g_tx = tx > 0 ? tx : 16;
+    g_ty = ty > 0 ? ty : 16;
+    g_halo = halo > 0 ? halo : 2;
+}
+
+static void runtime_init(void)
+{
+    if (g_ready) return;
+    int n = 0;
+    ZCHECK(hipGetDeviceCount(&n));
+    if (n <= 0) zpic_fatal("no HIP device available");
+    if (g_device < 0) ZCHECK(hipGetDevice(&g_device));
+    ZCHECK(hipSetDevice(g_device));
+    ZCHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_ready = 1;
+}
+
+zk_stream_t zpic_stream(void)
+{
+    runtime_init();
+    return g_stream;
+}
+
+void *zpic_dmalloc(size_t bytes)
+{
+    runtime_init();
+    void *p = NULL;
+    if (bytes == 0) bytes = 256;
+    ZCHECK(hipMalloc(&p, bytes));
+    return p;
+}
+
+void zpic_dfree(void *p)
+{
+    if (p) ZCHECK(hipFree(p));
+}
+
+void zpic_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, zpic_stream()));
+    ZCHECK(hipStreamSynchronize(zpic_stream()));
+}
+
+void zpic_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, zpic_stream()));
+    ZCHECK(hipStreamSynchronize(zpic_stream()));
+}
+
+void zpic_d2d_async(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, zpic_stream()));
+}
+
+void zpic_sync(void)
+{
+    if (g_ready) ZCHECK(hipStreamSynchronize(g_stream));
+}
+
+void zpic_synchronize(void) { zpic_sync(); }
+
+size_t zpic_grid_nodes(const int nx[2]) { return (size_t)(nx[0] + 3) * (size_t)(nx[1] + 3); }
+
+/* Tile width for one axis: the preferred width if it divides n, otherwise
+ * the divisor of n closest to it within [halo + 3, 64]. */
+static int pick_tile(int n, int want, int halo)
+{
+    if (want <= n && n % want == 0) return want;
+    int best = -1;
+    for (int d = halo + 3; d <= 64 && d <= n; d++) {
+        if (n % d) continue;
+        if (best < 0 || abs(d - want) < abs(best - want)) best = d;
+    }
+    if (best < 0 && n <= 64) best = n;
+    return best;
+}
+
+void zpic_tiles_for(int nx, int ny, zk_tiles *t)
+{
+    int h = g_halo;
+    int tx = -1, ty = -1;
+    for (; h >= 0; h--) {
+        tx = pick_tile(nx, g_tx, h);
+        ty = pick_tile(ny, g_ty, h);
+        if (tx >= h + 3 && ty >= h + 3) break;
+    }
+    if (tx < 3 || ty < 3 || h < 0)
+        zpic_fatal("grid %d x %d has no tile width in [3, 64] dividing it", nx, ny);
+    t->nx = nx;
+    t->ny = ny;
+    t->TX = tx;
+    t->TY = ty;
+    t->H = h;
+    t->WX = tx + 2 * h + 3;
+    t->WY = ty + 2 * h + 3;
+    t->ntx = nx / tx;
+    t->nty = ny / ty;
+    t->ntiles = t->ntx * t->nty;
+}
+
+/* ---- GPU timers: event pairs accumulated lazily ---- */
+#define ZPIC_EV_PAIRS 64
+static struct {
+    hipEvent_t ev[ZPIC_EV_PAIRS][2];
+    int used;
+    int open;
+    double total;
+    int created;
+} g_timer[ZPIC_NTIMERS];
+
+static void timer_drain(int w)
+{
+    if (!g_timer[w].used) return;
+    ZCHECK(hipEventSynchronize(g_timer[w].ev[g_timer[w].used - 1][1]));
+    for (int k = 0; k < g_timer[w].used; k++) {
+        float ms = 0.0f;
+        ZCHECK(hipEventElapsedTime(&ms, g_timer[w].ev[k][0], g_timer[w].ev[k][1]));
+        g_timer[w].total += 1e-3 * ms;
+    }
+    g_timer[w].used = 0;
+}
+
+void zpic_timer_begin(int w)
+{
+    hipStream_t s = zpic_stream();
+    if (!g_timer[w].created) {
+        for (int k = 0; k < ZPIC_EV_PAIRS; k++) {
+            ZCHECK(hipEventCreate(&g_timer[w].ev[k][0]));
+            ZCHECK(hipEventCreate(&g_timer[w].ev[k][1]));
+        }
+        g_timer[w].created = 1;
+    }
+    if (g_timer[w].used == ZPIC_EV_PAIRS) timer_drain(w);
+    ZCHECK(hipEventRecord(g_timer[w].ev[g_timer[w].used][0], s));
+    g_timer[w].open = 1;
+}
+
+void zpic_timer_end(int w)
+{
+    if (!g_timer[w].open) return;
+    ZCHECK(hipEventRecord(g_timer[w].ev[g_timer[w].used][1], zpic_stream()));
+    g_timer[w].used++;
+    g_timer[w].open = 0;
+}
+
+double zpic_timer_total(int w)
+{
+    if (g_ready) timer_drain(w);
+    return g_timer[w].total;
+}

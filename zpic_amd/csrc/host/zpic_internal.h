@@ -1,0 +1,79 @@
+/*
+ * zpic_internal.h - device-side state behind the em2d structs.
+ *
+ * Each t_species / t_emf / t_current carries a `dev` pointer to one of the
+ * structs below.  The device copy is authoritative once a step has run;
+ * the host buffers of the reference structs (part, E_buf, B_buf, J_buf)
+ * are mirrors, refreshed by the *_sync_host calls and uploaded again after
+ * the *_mark_host_dirty calls (or automatically before the first step).
+ */
+#ifndef ZPIC_INTERNAL_H
+#define ZPIC_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "simulation.h"
+#include "zpic_gpu.h"
+#include "zpic_rt.h"
+
+struct zpic_dev_species {
+    int ready;            /* device buffers allocated and filled */
+    int host_dirty;       /* host mirror edited: upload before the next push */
+    int host_valid;       /* host mirror equals the device contents */
+    int cap;              /* particle slots per buffer set */
+    int cur;              /* buffer set holding the particles */
+    zk_parts set[2];
+    int track_ids;
+    int64_t next_id;
+    zk_spec_ctr *ctr;     /* device counters */
+    zk_tiles t;
+    int *tile_off, *cnt, *cursor;
+    double *epart;
+    int epart_cap;
+    void *stage;          /* device AoS staging for uploads/downloads */
+    size_t stage_cap;
+    int np_upper;         /* upper bound of slots in use */
+    int tail_upper;       /* upper bound of unsorted slots */
+    int moved_since_sort; /* window moves since the last sort */
+    unsigned long long npush_retired; /* pushes counted before buffers were freed */
+    int64_t *host_ids;    /* ids in host-mirror order (when tracked) */
+    int host_ids_cap;
+    struct zpic_dev_species *next_live;
+    t_species *owner;
+};
+
+struct zpic_dev_current {
+    int ready;
+    int host_valid;
+    zk_tiles t;
+    float *J[2];
+    int jcur;
+    float *region;        /* per-tile current windows */
+    float *Jov;           /* raw current of out-of-window pushes */
+    int fresh;            /* no deposit since current_zero */
+};
+
+struct zpic_dev_emf {
+    int ready;
+    int host_dirty;
+    int host_valid;
+    float *E[2], *B[2];
+    int cur;
+    float *Ep, *Bp;       /* fields seen by particles (external fields on) */
+    float *Eext, *Bext;   /* device grids of custom external fields */
+    double *epartial, *energy6;
+    int nblk;
+};
+
+/* helpers shared by the host modules */
+void zpic_species_ensure(t_species *spec);
+void zpic_emf_ensure(t_emf *emf);
+void zpic_current_ensure(t_current *cur, const zk_tiles *t);
+const float *zpic_emf_particle_E(t_emf *emf);
+const float *zpic_emf_particle_B(t_emf *emf);
+const float *zpic_current_J(t_current *cur);
+
+void zpic_spec_sync_energy(t_species *s);
+
+#endif

@@ -1,0 +1,37 @@
+/*
+ * zpic_rt.h - HIP runtime services used by the host layer, in plain C.
+ *
+ * Only runtime.c includes the HIP headers; everything else goes through
+ * these functions, so the em2d headers (which define their own float3) and
+ * the HIP vector types never meet in one translation unit.  Every failure
+ * prints a message and exits with status -1, the reference's convention
+ * for fatal errors.
+ */
+#ifndef ZPIC_RT_H
+#define ZPIC_RT_H
+
+#include <stddef.h>
+
+#include "../kernels/zk.h"
+
+#define ZCHECK(call) zpic_check((call), #call, __FILE__, __LINE__)
+void zpic_check(int status, const char *what, const char *file, int line);
+void zpic_fatal(const char *fmt, ...);
+
+zk_stream_t zpic_stream(void);
+void *zpic_dmalloc(size_t bytes);
+void zpic_dfree(void *p);
+void zpic_h2d(void *dst, const void *src, size_t bytes);      /* synchronous */
+void zpic_d2h(void *dst, const void *src, size_t bytes);      /* synchronous */
+void zpic_d2d_async(void *dst, const void *src, size_t bytes); /* on the stream */
+void zpic_sync(void);
+void zpic_tiles_for(int nx, int ny, zk_tiles *t);
+size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */
+
+/* GPU timers for spec_time() / emf_time() */
+enum { ZPIC_T_SPEC = 0, ZPIC_T_EMF = 1, ZPIC_NTIMERS = 2 };
+void zpic_timer_begin(int which);
+void zpic_timer_end(int which);
+double zpic_timer_total(int which);
+
+#endif

@@ -1,0 +1,71 @@
+/*
+ * common.h - device helpers shared by the em2d HIP kernels (gfx950, wave64).
+ */
+#ifndef ZK_COMMON_H
+#define ZK_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zk.h"
+
+#define ZK_WAVE 64
+
+/* Blocks are dealt to the 8 XCDs round-robin.  Map block b to a work item
+ * so that each XCD receives one contiguous run of items: neighbouring tiles
+ * then share the halo lines in that XCD's L2. */
+__device__ inline int zk_xcd_item(int b, int n)
+{
+    const int nx = 8;
+    int x = b % nx, k = b / nx;
+    int base = n / nx, rem = n % nx;
+    return x * base + (x < rem ? x : rem) + k;
+}
+
+/* Floor modulo for periodic wrap. */
+__device__ inline int zk_wrap(int i, int n)
+{
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+/*
+ * Reserve one slot per active lane in counters[key], aggregating runs of
+ * equal keys held by consecutive lanes into a single atomic.  Lanes with
+ * key < 0 reserve nothing.  Must be called by all 64 lanes of the wave.
+ * Returns the reserved slot (counter value before the run + rank in run).
+ */
+__device__ inline int zk_run_reserve(int key, int *counters)
+{
+    const int lane = __lane_id();
+    int prev = __shfl_up(key, 1, ZK_WAVE);
+    bool head = (lane == 0) || (prev != key);
+    unsigned long long heads = __ballot(head);
+    unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    int myhead = 63 - __clzll(heads & upto);
+    unsigned long long after = heads & ~upto;
+    int next = after ? (__ffsll((long long)after) - 1) : ZK_WAVE;
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(&counters[key], next - lane);
+    base = __shfl(base, myhead, ZK_WAVE);
+    return base + lane - myhead;
+}
+
+/* Sum of a double over the block (blockDim.x a multiple of 64, <= 1024).
+ * Result valid in thread 0. */
+__device__ inline double zk_block_sum(double v, double *lds32)
+{
+    for (int o = ZK_WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, ZK_WAVE);
+    const int w = threadIdx.x / ZK_WAVE, nw = blockDim.x / ZK_WAVE;
+    if ((threadIdx.x & (ZK_WAVE - 1)) == 0) lds32[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nw; k++) s += lds32[k];
+    __syncthreads();
+    return s;
+}
+
+#define ZK_LAUNCH_CHECK() return (int)hipGetLastError()
+
+#endif

@@ -1,0 +1,271 @@
+/*
+ * fields.hip - Yee field solver and field bookkeeping (gfx950).
+ *
+ * zk_yee fuses the three half/whole steps of the reference emf_advance
+ * (em2d/emf.c:688-698): B += dt/2 curl E over i,j in [-1,n] (yee_b,
+ * :500-522), E += dt (curl B - J) over [0,n+1] (yee_e, :531-562), and
+ * B += dt/2 curl E again, into one pass over LDS tiles.  Each node is
+ * computed with the reference's expressions and update ranges, so with the
+ * same J the result is bit-identical.  Inputs and outputs are distinct
+ * buffers (the host swaps them).
+ *
+ * zk_field_guards copies guard cells from their periodic images
+ * (emf_update_gc, :573-637); zk_move_window shifts the grid one cell left
+ * and clears the last columns (emf_move_window, :648-675); zk_add_ext
+ * forms the fields seen by particles (emf_update_part_fld, :838-914);
+ * zk_emf_energy sums E_c^2 and B_c^2 in FP64 (emf_get_energy, :729-750);
+ * zk_charge deposits the CIC charge density (spec_deposit_charge,
+ * em2d/particles.c:1289-1324, before its boundary fold).
+ */
+#include "common.h"
+
+namespace {
+
+constexpr int BX = 32, BY = 16;  /* output nodes per tile */
+constexpr int LX = BX + 3, LY = BY + 3;
+
+struct YeeK {
+    float dtx2, dty2;  /* (dt/2)/dx, (dt/2)/dy */
+    float dtx, dty, dt;
+};
+
+__global__ void __launch_bounds__(256) yee_kernel(const float *__restrict__ E, const float *__restrict__ B,
+                                                  const float *__restrict__ J, float *__restrict__ Eo,
+                                                  float *__restrict__ Bo, int nx, int ny, int nrow, YeeK k)
+{
+    __shared__ float e[3][LY][LX], b[3][LY][LX], jj[3][LY][LX];
+    __shared__ float b1[3][LY][LX], e1[3][LY][LX];
+    const int bw = nx + 3, bh = ny + 3;
+    const int bi0 = blockIdx.x * BX, bj0 = blockIdx.y * BY;
+    const int tid = threadIdx.x;
+    /* local (u, v) <-> buffer node (bi0 - 1 + u, bj0 - 1 + v) */
+    for (int m = tid; m < LX * LY; m += blockDim.x) {
+        const int u = m % LX, v = m / LX;
+        const int bi = bi0 - 1 + u, bj = bj0 - 1 + v;
+        const bool in = bi >= 0 && bi < bw && bj >= 0 && bj < bh;
+        const size_t g = 3 * ((size_t)bi + (size_t)bj * nrow);
+        for (int c = 0; c < 3; c++) {
+            e[c][v][u] = in ? E[g + c] : 0.0f;
+            b[c][v][u] = in ? B[g + c] : 0.0f;
+            jj[c][v][u] = in ? J[g + c] : 0.0f;
+        }
+    }
+    __syncthreads();
+    /* B1 = B + dt/2 * (-curl E) on u in [0, LX-1), v in [0, LY-1) */
+    for (int m = tid; m < (LX - 1) * (LY - 1); m += blockDim.x) {
+        const int u = m % (LX - 1), v = m / (LX - 1);
+        const int i = bi0 - 2 + u, j = bj0 - 2 + v; /* raw indices */
+        float bx = b[0][v][u], by = b[1][v][u], bz = b[2][v][u];
+        if (i >= -1 && i <= nx && j >= -1 && j <= ny) {
+            bx += (-k.dty2 * (e[2][v + 1][u] - e[2][v][u]));
+            by += (k.dtx2 * (e[2][v][u + 1] - e[2][v][u]));
+            bz += (-k.dtx2 * (e[1][v][u + 1] - e[1][v][u]) + k.dty2 * (e[0][v + 1][u] - e[0][v][u]));
+        }
+        b1[0][v][u] = bx;
+        b1[1][v][u] = by;
+        b1[2][v][u] = bz;
+    }
+    __syncthreads();
+    /* E1 = E + dt * (curl B1 - J) on u in [1, LX-1), v in [1, LY-1) */
+    for (int m = tid; m < (LX - 2) * (LY - 2); m += blockDim.x) {
+        const int u = 1 + m % (LX - 2), v = 1 + m / (LX - 2);
+        const int i = bi0 - 2 + u, j = bj0 - 2 + v;
+        float ex = e[0][v][u], ey = e[1][v][u], ez = e[2][v][u];
+        if (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1) {
+            ex += (+k.dty * (b1[2][v][u] - b1[2][v - 1][u])) - k.dt * jj[0][v][u];
+            ey += (-k.dtx * (b1[2][v][u] - b1[2][v][u - 1])) - k.dt * jj[1][v][u];
+            ez += (+k.dtx * (b1[1][v][u] - b1[1][v][u - 1]) - k.dty * (b1[0][v][u] - b1[0][v - 1][u])) -
+                  k.dt * jj[2][v][u];
+        }
+        e1[0][v][u] = ex;
+        e1[1][v][u] = ey;
+        e1[2][v][u] = ez;
+    }
+    __syncthreads();
+    /* B2 = B1 + dt/2 * (-curl E1) and outputs on u in [1, BX+1), v in [1, BY+1) */
+    for (int m = tid; m < BX * BY; m += blockDim.x) {
+        const int u = 1 + m % BX, v = 1 + m / BX;
+        const int bi = bi0 - 1 + u, bj = bj0 - 1 + v;
+        if (bi >= bw || bj >= bh) continue;
+        const int i = bi - 1, j = bj - 1;
+        float bx = b1[0][v][u], by = b1[1][v][u], bz = b1[2][v][u];
+        if (i >= -1 && i <= nx && j >= -1 && j <= ny) {
+            bx += (-k.dty2 * (e1[2][v + 1][u] - e1[2][v][u]));
+            by += (k.dtx2 * (e1[2][v][u + 1] - e1[2][v][u]));
+            bz += (-k.dtx2 * (e1[1][v][u + 1] - e1[1][v][u]) + k.dty2 * (e1[0][v + 1][u] - e1[0][v][u]));
+        }
+        const size_t g = 3 * ((size_t)bi + (size_t)bj * nrow);
+        Bo[g + 0] = bx;
+        Bo[g + 1] = by;
+        Bo[g + 2] = bz;
+        Eo[g + 0] = e1[0][v][u];
+        Eo[g + 1] = e1[1][v][u];
+        Eo[g + 2] = e1[2][v][u];
+    }
+}
+
+/* Guard nodes take the value of their image: x (unless the window moves)
+ * and y.  One thread per guard node. */
+__global__ void guard_kernel(float *E, float *B, int nx, int ny, int nrow, int periodic_x)
+{
+    const int bw = nx + 3;
+    const int nrows_g = 3 * bw;       /* guard rows: j = -1, ny, ny+1 */
+    const int ncols_g = 3 * ny;       /* guard columns of interior rows */
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j;
+    if (k < nrows_g) {
+        i = k % bw - 1;
+        const int r = k / bw;
+        j = (r == 0) ? -1 : ny + r - 1;
+    } else if (k < nrows_g + ncols_g) {
+        const int q = k - nrows_g;
+        const int c = q % 3;
+        j = q / 3;
+        i = (c == 0) ? -1 : nx + c - 1;
+    } else {
+        return;
+    }
+    const int ci = periodic_x ? zk_wrap(i, nx) : i;
+    const int cj = zk_wrap(j, ny);
+    if (ci == i && cj == j) return;
+    const size_t d = 3 * ((size_t)(i + 1) + (size_t)(j + 1) * nrow);
+    const size_t s = 3 * ((size_t)(ci + 1) + (size_t)(cj + 1) * nrow);
+    for (int c = 0; c < 3; c++) {
+        E[d + c] = E[s + c];
+        B[d + c] = B[s + c];
+    }
+}
+
+__global__ void move_window_kernel(const float *E, const float *B, float *Eo, float *Bo, int nx, int ny, int nrow)
+{
+    const int bw = nx + 3, bh = ny + 3;
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= bw * bh) return;
+    const int i = node % bw - 1;
+    const size_t d = 3 * (size_t)node;
+    if (i >= nx - 1) {
+        for (int c = 0; c < 3; c++) Eo[d + c] = Bo[d + c] = 0.0f;
+    } else {
+        for (int c = 0; c < 3; c++) {
+            Eo[d + c] = E[d + 3 + c];
+            Bo[d + c] = B[d + 3 + c];
+        }
+    }
+}
+
+__global__ void add_ext_kernel(const float *F, const float *ext, float ux, float uy, float uz, float *out,
+                               size_t nodes)
+{
+    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < nodes; n += (size_t)gridDim.x * blockDim.x) {
+        float a0 = ext ? ext[3 * n + 0] : ux;
+        float a1 = ext ? ext[3 * n + 1] : uy;
+        float a2 = ext ? ext[3 * n + 2] : uz;
+        out[3 * n + 0] = F[3 * n + 0] + a0;
+        out[3 * n + 1] = F[3 * n + 1] + a1;
+        out[3 * n + 2] = F[3 * n + 2] + a2;
+    }
+}
+
+__global__ void energy_partial(const float *E, const float *B, int nx, int ny, int nrow, double *partial)
+{
+    __shared__ double red[16];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    const int cells = nx * ny;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+        const int i = c % nx, j = c / nx;
+        const size_t g = 3 * ((size_t)(i + 1) + (size_t)(j + 1) * nrow);
+        for (int q = 0; q < 3; q++) {
+            const float e = E[g + q], b = B[g + q];
+            s[q] += (double)(e * e);
+            s[3 + q] += (double)(b * b);
+        }
+    }
+    for (int q = 0; q < 6; q++) {
+        const double v = zk_block_sum(s[q], red);
+        if (threadIdx.x == 0) partial[6 * blockIdx.x + q] = v;
+    }
+}
+
+__global__ void energy_final(const double *partial, int nblk, double *out6, double scale)
+{
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += partial[6 * b + threadIdx.x];
+        out6[threadIdx.x] = s * scale;
+    }
+}
+
+__global__ void charge_kernel(zk_parts p, const zk_spec_ctr *ctr, float q, int nx, float *rho)
+{
+    const int n = ctr->np_total;
+    const int nrow = nx + 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int ix = p.ix[i];
+        if (ix == ZK_DEAD) continue;
+        const int c = ix + nrow * p.iy[i];
+        const float wx = p.x[i], wy = p.y[i];
+        unsafeAtomicAdd(&rho[c], (1.0f - wx) * (1.0f - wy) * q);
+        unsafeAtomicAdd(&rho[c + 1], (wx) * (1.0f - wy) * q);
+        unsafeAtomicAdd(&rho[c + nrow], (1.0f - wx) * (wy) * q);
+        unsafeAtomicAdd(&rho[c + 1 + nrow], (wx) * (wy) * q);
+    }
+}
+
+} // namespace
+
+extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo, int nx, int ny,
+                             int nrow, float dt, float dx, float dy, zk_stream_t s)
+{
+    YeeK k;
+    const float dt2 = dt / 2.0f;
+    k.dtx2 = dt2 / dx;
+    k.dty2 = dt2 / dy;
+    k.dtx = dt / dx;
+    k.dty = dt / dy;
+    k.dt = dt;
+    dim3 grid((nx + 3 + BX - 1) / BX, (ny + 3 + BY - 1) / BY);
+    hipLaunchKernelGGL(yee_kernel, grid, dim3(256), 0, s, E, B, J, Eo, Bo, nx, ny, nrow, k);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x, zk_stream_t s)
+{
+    const int n = 3 * (nx + 3) + 3 * ny;
+    hipLaunchKernelGGL(guard_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, B, nx, ny, nrow, periodic_x);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_move_window(const float *E, const float *B, float *Eo, float *Bo, int nx, int ny, int nrow,
+                                     zk_stream_t s)
+{
+    const int nodes = (nx + 3) * (ny + 3);
+    hipLaunchKernelGGL(move_window_kernel, dim3((nodes + 255) / 256), dim3(256), 0, s, E, B, Eo, Bo, nx, ny, nrow);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_add_ext(const float *F, const float *ext, float ux, float uy, float uz, float *out,
+                                 size_t nodes, zk_stream_t s)
+{
+    size_t blocks = (nodes + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(add_ext_kernel, dim3((unsigned)blocks), dim3(256), 0, s, F, ext, ux, uy, uz, out, nodes);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_emf_energy(const float *E, const float *B, int nx, int ny, int nrow, double *partial,
+                                    int nblk, double *out6, double scale, zk_stream_t s)
+{
+    hipLaunchKernelGGL(energy_partial, dim3(nblk), dim3(256), 0, s, E, B, nx, ny, nrow, partial);
+    hipLaunchKernelGGL(energy_final, dim3(1), dim3(64), 0, s, partial, nblk, out6, scale);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, float q, int nx, float *rho,
+                                zk_stream_t s)
+{
+    long long b = (np_upper + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(charge_kernel, dim3((unsigned)b), dim3(256), 0, s, *p, ctr, q, nx, rho);
+    ZK_LAUNCH_CHECK();
+}

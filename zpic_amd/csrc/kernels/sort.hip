@@ -1,0 +1,185 @@
+/*
+ * sort.hip - counting sort of the particle arrays by tile (gfx950).
+ *
+ * Plays the role of the reference spec_sort (em2d/particles.c:942-1007),
+ * which orders particles by cell to keep memory access local.  Here the key
+ * is the tile (TX x TY cells) that the push kernel assigns one workgroup
+ * to, so after the sort every tile owns one contiguous range of slots.
+ * Removed particles (ZK_DEAD) are dropped.  Order inside a tile follows
+ * the atomics and is not stable; the particle order is not part of the
+ * em2d contract (the reference itself reorders on sort and on removal).
+ *
+ * Three launches: count (per-tile histogram), scan (tile offsets, one
+ * workgroup), scatter (write every live particle to its tile range).  Count
+ * and scatter reserve slots with one atomic per run of equal keys in a
+ * wave, so a tile-ordered input costs about one atomic per wave.
+ */
+#include "common.h"
+
+namespace {
+
+__device__ inline int tile_key(int ix, int iy, const zk_tiles &t)
+{
+    if (ix == ZK_DEAD) return -1;
+    return ix / t.TX + (iy / t.TY) * t.ntx;
+}
+
+__global__ void count_kernel(const int *__restrict__ ix, const int *__restrict__ iy, const zk_spec_ctr *ctr,
+                             int *cnt, zk_tiles t)
+{
+    const int n = ctr->np_total;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        const int key = (i < n) ? tile_key(ix[i], iy[i], t) : -1;
+        zk_run_reserve(key, cnt);
+    }
+}
+
+/* Exclusive scan of the tile counts into off[0..ntiles], cursor = off;
+ * counts are cleared for the next sort.  One workgroup of 1024 threads. */
+__global__ void __launch_bounds__(1024) scan_kernel(int *cnt, int *off, int *cursor, zk_spec_ctr *ctr, int ntiles)
+{
+    __shared__ int part[1024];
+    const int per = (ntiles + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per, e = min(ntiles, b + per);
+    int s = 0;
+    for (int i = b; i < e; i++) s += cnt[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        int v = (threadIdx.x >= (unsigned)o) ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - s;
+    for (int i = b; i < e; i++) {
+        const int c = cnt[i];
+        off[i] = run;
+        cursor[i] = run;
+        cnt[i] = 0;
+        run += c;
+    }
+    if (threadIdx.x == blockDim.x - 1) {
+        off[ntiles] = part[threadIdx.x];
+        ctr->np_total = part[threadIdx.x];
+        ctr->n_sorted = part[threadIdx.x];
+        ctr->n_dead = 0;
+    }
+}
+
+__global__ void snapshot_kernel(const zk_spec_ctr *ctr, int *n_old)
+{
+    *n_old = ctr->np_total;
+}
+
+__global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, int *cursor, zk_tiles t)
+{
+    const int n = *n_old;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        int ix = 0, iy = 0, key = -1;
+        if (i < n) {
+            ix = src.ix[i];
+            iy = src.iy[i];
+            key = tile_key(ix, iy, t);
+        }
+        const int d = zk_run_reserve(key, cursor);
+        if (key >= 0) {
+            dst.ix[d] = ix;
+            dst.iy[d] = iy;
+            dst.x[d] = src.x[i];
+            dst.y[d] = src.y[i];
+            dst.ux[d] = src.ux[i];
+            dst.uy[d] = src.uy[i];
+            dst.uz[d] = src.uz[i];
+            if (src.id) dst.id[d] = src.id[i];
+        }
+    }
+}
+
+/* t_part AoS (28 bytes: ix, iy, x, y, ux, uy, uz) <-> SoA */
+__global__ void unpack_kernel(const int *__restrict__ aos, int n, zk_parts dst, int64_t id0, zk_spec_ctr *ctr,
+                              int append)
+{
+    const int base = append ? ctr->np_total : 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int *r = aos + 7 * (size_t)i;
+        const int d = base + i;
+        dst.ix[d] = r[0];
+        dst.iy[d] = r[1];
+        dst.x[d] = __int_as_float(r[2]);
+        dst.y[d] = __int_as_float(r[3]);
+        dst.ux[d] = __int_as_float(r[4]);
+        dst.uy[d] = __int_as_float(r[5]);
+        dst.uz[d] = __int_as_float(r[6]);
+        if (dst.id) dst.id[d] = id0 + i;
+    }
+}
+
+__global__ void unpack_commit(zk_spec_ctr *ctr, int n, int append)
+{
+    if (append) {
+        ctr->np_total += n;
+    } else {
+        ctr->np_total = n;
+        ctr->n_sorted = 0;
+        ctr->n_dead = 0;
+    }
+}
+
+__global__ void pack_kernel(zk_parts src, int n, int *__restrict__ aos)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int *r = aos + 7 * (size_t)i;
+        r[0] = src.ix[i];
+        r[1] = src.iy[i];
+        r[2] = __float_as_int(src.x[i]);
+        r[3] = __float_as_int(src.y[i]);
+        r[4] = __float_as_int(src.ux[i]);
+        r[5] = __float_as_int(src.uy[i]);
+        r[6] = __float_as_int(src.uz[i]);
+    }
+}
+
+unsigned grid_for(long long n, int bs, unsigned cap)
+{
+    long long b = (n + bs - 1) / bs;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+} // namespace
+
+extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *cnt, int *off,
+                              int *cursor, const zk_tiles *t, int np_upper, zk_stream_t s)
+{
+    const unsigned g = grid_for(np_upper, 256, 8192);
+    /* the slot count in use before the scan is kept in off[ntiles + 1] */
+    int *n_old = off + t->ntiles + 1;
+    hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, n_old);
+    hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, ctr, cnt, *t);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles);
+    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, *t);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_unpack(const void *aos, int n, const zk_parts *dst, int64_t id0, zk_spec_ctr *ctr,
+                                int append, zk_stream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const int *)aos, n, *dst,
+                           id0, ctr, append);
+    hipLaunchKernelGGL(unpack_commit, dim3(1), dim3(1), 0, s, ctr, n, append);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_pack(const zk_parts *src, int n, void *aos, zk_stream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(pack_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, *src, n, (int *)aos);
+    ZK_LAUNCH_CHECK();
+}

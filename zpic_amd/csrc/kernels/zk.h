@@ -1,0 +1,121 @@
+/*
+ * zk.h - launchers of the em2d HIP kernels (gfx950), callable from C.
+ *
+ * Every launcher takes plain device pointers, sizes and scalars in a small
+ * argument struct and a HIP stream; it returns the int of the
+ * launch (0 on success).  The host layer (zpic_amd/csrc/host) owns all allocations.
+ *
+ * Grids are float3 arrays of (nx+3) x (ny+3) nodes including guard cells
+ * (one lower, two upper guards per axis), row stride nrow = nx + 3, and are
+ * addressed from the buffer origin: node (i, j), i,j >= -1, sits at
+ * (i + 1) + (j + 1) * nrow.  Particles are stored as seven separate arrays
+ * (ix, iy, x, y, ux, uy, uz), grouped by tile.
+ */
+#ifndef ZK_H
+#define ZK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+/* HIP stream handle (same type as hipStream_t) and launch status (a
+ * int value, 0 on success); kept opaque so that C code including
+ * the em2d headers need not include the HIP headers, whose vector types
+ * clash with the em2d float3. */
+typedef struct ihipStream_t *zk_stream_t;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cell index that marks a removed particle (absorbing boundary). */
+#define ZK_DEAD (-0x40000000)
+
+/* Per-species counters kept in device memory. */
+typedef struct zk_spec_ctr {
+    int np_total;               /* slots in use: tile ranges + unsorted tail */
+    int n_sorted;               /* end of the tile ranges                     */
+    int n_dead;                 /* removed particles still occupying slots    */
+    int n_overflow;             /* pushes that left the tile window this step */
+    unsigned long long npush;   /* accumulated particle pushes                */
+    double energy;              /* kinetic energy of the last push            */
+} zk_spec_ctr;
+
+/* Particle arrays of one buffer set. */
+typedef struct zk_parts {
+    int *ix, *iy;
+    float *x, *y, *ux, *uy, *uz;
+    int64_t *id;                /* optional identities (NULL when untracked) */
+} zk_parts;
+
+/* Tile geometry shared by the push, the sort and the current reduction.
+ * Tiles are TX x TY cells; nx % TX == 0, ny % TY == 0.  A tile's window is
+ * its cells plus H cells of drift margin plus the interpolation/deposit
+ * stencil (1 below, 2 above): WX = TX + 2H + 3 nodes, WY likewise. */
+typedef struct zk_tiles {
+    int nx, ny, TX, TY, H, WX, WY, ntx, nty, ntiles;
+} zk_tiles;
+
+typedef struct zk_push_args {
+    zk_parts p;
+    const int *tile_off;        /* [ntiles + 1] */
+    zk_spec_ctr *ctr;
+    const float *E, *B;         /* fields seen by particles, buffer origin */
+    int nrow;
+    float *region;              /* [ntiles][3][WY][WX] current windows */
+    int fresh;                  /* 1: overwrite region, 0: accumulate    */
+    float *Jov;                 /* raw current of out-of-window pushes  */
+    double *epart;              /* energy partials, one per block        */
+    zk_tiles t;
+    int moving_window;          /* x absorbing + window shift           */
+    int shift;                  /* 1 when the window moves this step    */
+    int tail_blocks;            /* extra blocks for the unsorted tail    */
+    float tem, dt_dx, dt_dy, qnx, qny, q;
+} zk_push_args;
+
+int zk_fill_f32(float *p, size_t n, float v, zk_stream_t s);
+int zk_push(const zk_push_args *a, zk_stream_t s);
+int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int nparts,
+                            double qm, double dx0, double dx1, zk_stream_t s);
+
+/* tile counting sort: src -> dst; off has ntiles + 2 entries, cnt and cursor ntiles */
+int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr,
+                   int *cnt, int *off, int *cursor, const zk_tiles *t,
+                   int np_upper, zk_stream_t s);
+/* AoS staging <-> SoA */
+int zk_unpack(const void *aos, int n, const zk_parts *dst, int64_t id0,
+                     zk_spec_ctr *ctr, int append, zk_stream_t s);
+int zk_pack(const zk_parts *src, int n, void *aos, zk_stream_t s);
+
+/* current: window reduction + raw overflow -> folded J; smoothing passes */
+typedef struct zk_reduce_args {
+    const float *region;
+    const float *Jov;
+    float *J;
+    int nrow;
+    zk_tiles t;
+    int periodic_x;
+} zk_reduce_args;
+int zk_current_reduce(const zk_reduce_args *a, zk_stream_t s);
+int zk_smooth(const float *in, float *out, int nx, int ny, int nrow, int axis,
+                     float sa, float sb, int periodic_x, zk_stream_t s);
+
+/* fields */
+int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo,
+                  int nx, int ny, int nrow, float dt, float dx, float dy, zk_stream_t s);
+int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x,
+                           zk_stream_t s);
+int zk_move_window(const float *E, const float *B, float *Eo, float *Bo, int nx,
+                          int ny, int nrow, zk_stream_t s);
+/* out = F + ext[node] when ext is given, else F + (ux,uy,uz), over every node */
+int zk_add_ext(const float *F, const float *ext, float ux, float uy, float uz,
+                      float *out, size_t nodes, zk_stream_t s);
+int zk_emf_energy(const float *E, const float *B, int nx, int ny, int nrow,
+                         double *partial, int nblk, double *out6, double scale, zk_stream_t s);
+int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, float q,
+                     int nx, float *rho, zk_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
