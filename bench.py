@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the em2d time step on one MI355X.
+
+Workload (the large Weibel configuration): 2048 x 2048 periodic grid,
+dx = dy = 0.1, dt = 0.07, two counter-streaming electron species (m_q = -1,
+u_z = +-0.6, u_th = 0.1), 4 x 4 particles per cell each (134,217,728
+particles), tile sort every 20 steps, one binomial smoothing pass along x
+and y.  The default run is the full 1000-step simulation: 10 untimed
+warm-up steps, then 990 timed steps.
+
+Prints one JSON line: particle-steps/s over the timed steps, the HBM
+roofline of the dominant kernel (push + deposit, timed with HIP events on
+the library's stream), and a CPU baseline: the reference em2d built with
+its own flags (oracle/_ref/ref_bench), run on the host cores over a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def ensure_built() -> None:
+    if not os.path.exists(os.path.join(ROOT, "zpic_amd", "libzpic_em2d.so")):
+        import __graft_entry__
+
+        __graft_entry__.build()
+
+
+def cpu_baseline(sample_nx: int, ppc: int, steps: int, nproc: int) -> dict | None:
+    """Reference em2d (its own -Ofast build) on `nproc` host cores at once,
+    each running the sample; aggregate particle-steps/s."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    kind = "reference"
+    if not os.path.exists(exe):
+        exe = os.path.join(ROOT, "oracle", "_build", "oracle_bench")
+        kind = "port"
+    if not os.path.exists(exe):
+        return None
+    args = [exe, str(sample_nx), str(sample_nx), str(ppc), str(ppc), str(steps), "20", "1", "0.1", "0.07", "0.6",
+            "0.1", "-1"]
+    t0 = time.time()
+    procs = [subprocess.Popen(args, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for _ in range(nproc)]
+    outs = [p.communicate()[0] for p in procs]
+    wall = time.time() - t0
+    rates = []
+    for o in outs:
+        try:
+            rates.append(json.loads(o.strip().splitlines()[-1])["particle_steps_per_s"])
+        except Exception:
+            return None
+    npart = 2 * sample_nx * sample_nx * ppc * ppc
+    return {"value": float(sum(rates)), "unit": "particle-steps/s", "cores": nproc, "kind": kind,
+            "single_core_value": float(sorted(rates)[len(rates) // 2]),
+            "sample": (f"same physics on a {sample_nx}x{sample_nx} grid ({npart} particles), {steps} steps, "
+                       f"{nproc} independent processes on {nproc} host cores, {wall:.1f} s wall")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=990)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ppc", type=int, default=4)
+    ap.add_argument("--n-sort", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-nx", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the time step would shard by domain decomposition, not implemented here)", file=sys.stderr)
+        return 2
+
+    ensure_built()
+    import ctypes as C
+
+    from zpic_amd import _lib
+    from zpic_amd import em2d
+
+    lib = _lib.load()
+    nx = args.nx
+    ppc = args.ppc
+    dx, dt, uz, uth = 0.1, 0.07, 0.6, 0.1
+    log(f"bench: building Weibel {nx}x{nx}, 2 species x {ppc}x{ppc} ppc")
+    t_init = time.time()
+    sp = [em2d.Species("electrons", -1.0, [ppc, ppc], ufl=[0.0, 0.0, uz], uth=[uth] * 3, n_sort=args.n_sort),
+          em2d.Species("electrons2", -1.0, [ppc, ppc], ufl=[0.0, 0.0, -uz], uth=[uth] * 3, n_sort=args.n_sort)]
+    sim = em2d.Simulation([nx, nx], [nx * dx, nx * dx], dt, species=sp,
+                          smooth=em2d.Smooth(xtype="binomial", ytype="binomial", xlevel=1, ylevel=1))
+    npart = sum(s._s.np for s in sp)
+    log(f"bench: init {time.time() - t_init:.1f} s, {npart} particles; {args.warmup} warm-up steps")
+    for _ in range(args.warmup):
+        sim.iter()
+    lib.zpic_synchronize()
+
+    lib.zpic_kernel_timing_reset()
+    lib.zpic_kernel_timing(1)
+    log(f"bench: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.iter()
+        if (k + 1) % 200 == 0:
+            log(f"bench: queued {k + 1} steps")
+    lib.zpic_synchronize()
+    t1 = time.perf_counter()
+    lib.zpic_kernel_timing(0)
+    elapsed = t1 - t0
+    launches = C.c_longlong(0)
+    push_s = lib.zpic_kernel_time(0, C.byref(launches))
+    kt = {}
+    for k, name in enumerate(["push", "sort", "current", "field"]):
+        n = C.c_longlong(0)
+        tot = lib.zpic_kernel_time(k, C.byref(n))
+        kt[name] = {"ms_per_launch": 1e3 * tot / max(n.value, 1), "launches": n.value}
+    overflow = [lib.spec_overflow_count(s._ptr) for s in sp]
+    sim.close()
+
+    value = npart * args.steps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    cells = nx * nx
+    np_species = npart // 2
+    bytes_per_launch = 56.0 * np_species + 36.0 * cells
+    avg_push = push_s / max(launches.value, 1)
+    achieved = bytes_per_launch / avg_push / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "push_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            tj = json.load(open(tfile))
+            if tj.get("nx") == nx and tj.get("ppc") == ppc:
+                traffic = tj.get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        nproc = args.cpu_procs or min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline on {nproc} host cores")
+        cpu = cpu_baseline(args.cpu_nx, ppc, args.cpu_steps, nproc)
+
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Weibel initial state from the em2d injection (MWC + Box-Muller)",
+        "config": {"workload": (f"em2d Weibel {nx}x{nx} periodic, dx=dy=0.1, dt=0.07, 2 electron species "
+                                f"u_z=+-0.6 u_th=0.1, {ppc}x{ppc} ppc each ({npart} particles), "
+                                f"sort every {args.n_sort} steps, 1-pass binomial smoothing x and y"),
+                   "nx": nx, "ppc": ppc, "particles": npart, "steps_total": args.warmup + args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "push_kernel (push + deposit, one launch per species per step)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": 1e3 * avg_push},
+        "cpu_baseline": cpu,
+        "kernels": kt,
+        "overflow_pushes": overflow,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,6 +34,12 @@ void zpic_set_device(int device);
  * the largest divisor <= 64 that leaves room for the margin. */
 void zpic_set_tiling(int tx, int ty, int halo);
 
+/* Drop the deviate the normal generator keeps for its next call.
+ * set_rand_seed keeps it, as the reference does (em2d/random.c:25-29,
+ * 69-70); clearing it lets independent runs in one process start from
+ * identical generator states. */
+void zpic_rand_clear(void);
+
 /* Block until all queued GPU work is done. */
 void zpic_synchronize(void);
 
@@ -54,9 +60,27 @@ void emf_mark_host_dirty(t_emf *emf);
 void spec_track_ids(t_species *spec);
 void spec_get_ids(t_species *spec, int64_t *ids);
 
-/* Number of pushes of the last step that left their tile window (took the
- * global-memory path); a diagnostic of the tile margin. */
+/* Pushes, accumulated since the species was created, that found their
+ * particle outside the tile window (and took the global-memory path); a
+ * diagnostic of the tile margin. */
 int spec_overflow_count(t_species *spec);
+
+/* Per-kernel GPU time, measured with HIP events on the library's stream
+ * around each launch while enabled.  Classes: 0 push and deposit, 1 tile
+ * sort, 2 current reduction and smoothing, 3 field solver. */
+enum zpic_kernel_class { ZPIC_K_PUSH = 0, ZPIC_K_SORT = 1, ZPIC_K_CURRENT = 2, ZPIC_K_FIELD = 3 };
+void zpic_kernel_timing(int enable);
+double zpic_kernel_time(int kclass, long long *launches); /* seconds */
+void zpic_kernel_timing_reset(void);
+
+/* Adapter for field callbacks that cannot return a float3 by value (e.g.
+ * from Python): set E_custom/B_custom to zpic_field_callback and the data
+ * pointer to a zpic_field_cb whose fn writes the three components. */
+typedef struct zpic_field_cb {
+    void (*fn)(int ix, float dx, int iy, float dy, float *out3, void *user);
+    void *user;
+} zpic_field_cb;
+float3 zpic_field_callback(int ix, float dx, int iy, float dy, void *data);
 
 #ifdef __cplusplus
 }

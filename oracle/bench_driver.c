@@ -9,7 +9,9 @@
  * taken from the command line), runs `steps` calls of sim_iter and prints
  * one JSON line with the push count and the timers.
  *
- * usage: bench nx ny ppcx ppcy steps n_sort smooth_level [dx dt uz uth]
+ * usage: bench nx ny ppcx ppcy steps n_sort smooth_level [dx dt uz uth m_q2]
+ * (m_q2 is the mass-to-charge ratio of the second, counter-drifting species;
+ * +1 as in the deck, -1 for two electron populations)
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -20,7 +22,7 @@
 int main(int argc, char **argv)
 {
     if (argc < 8) {
-        fprintf(stderr, "usage: %s nx ny ppcx ppcy steps n_sort smooth_level [dx dt uz uth]\n", argv[0]);
+        fprintf(stderr, "usage: %s nx ny ppcx ppcy steps n_sort smooth_level [dx dt uz uth m_q2]\n", argv[0]);
         return 2;
     }
     int nx[2] = {atoi(argv[1]), atoi(argv[2])};
@@ -32,6 +34,7 @@ int main(int argc, char **argv)
     float dt = argc > 9 ? (float)atof(argv[9]) : 0.07f;
     float uz = argc > 10 ? (float)atof(argv[10]) : 0.6f;
     float uth = argc > 11 ? (float)atof(argv[11]) : 0.1f;
+    float mq2 = argc > 12 ? (float)atof(argv[12]) : 1.0f;
     float box[2] = {nx[0] * h, nx[1] * h};
 
     uint64_t ti0 = timer_ticks();
@@ -40,7 +43,7 @@ int main(int argc, char **argv)
     float th[3] = {uth, uth, uth};
     spec_new(&sp[0], "electrons", -1.0f, ppc, ufl, th, nx, box, dt, NULL);
     ufl[2] = -uz;
-    spec_new(&sp[1], "positrons", +1.0f, ppc, ufl, th, nx, box, dt, NULL);
+    spec_new(&sp[1], mq2 > 0 ? "positrons" : "electrons2", mq2, ppc, ufl, th, nx, box, dt, NULL);
     sp[0].n_sort = n_sort;
     sp[1].n_sort = n_sort;
     t_simulation sim;

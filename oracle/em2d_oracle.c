@@ -80,6 +80,10 @@ void set_rand_seed(uint32_t w, uint32_t z)
     rng_z = z;
 }
 
+/* test extension: drop the cached deviate so independent runs in one
+ * process start from the same generator state */
+void orc_rng_reset(void) { rng_have_spare = 0; }
+
 static uint32_t mwc32(void)
 {
     rng_z = 36969u * (rng_z & 65535u) + (rng_z >> 16);

@@ -41,6 +41,11 @@ class Em2dRun:
         box = (C.c_float * 2)(*cfg["box"])
         dt = C.c_float(cfg["dt"])
         seed = cfg.get("seed", [12345, 67890])
+        # start from an empty deviate cache where the library can clear it
+        # (the reference's set_rand_seed keeps it, em2d/random.c:25-29)
+        for clear in ("orc_rng_reset", "zpic_rand_clear"):
+            if hasattr(lib, clear):
+                getattr(lib, clear)()
         lib.set_rand_seed(seed[0], seed[1])
         # the species array is handed to sim_new, which frees it in sim_delete
         libc = C.CDLL(None)
@@ -61,6 +66,11 @@ class Em2dRun:
                 d.type = A.DENSITY[dd.get("type", "uniform")]
                 d.start = dd.get("start", 0.0)
                 d.end = dd.get("end", 0.0)
+                for ax in ("x", "y"):
+                    if f"custom_{ax}" in dd:
+                        cb = A.DENSITY_FN(lambda v, data, fn=dd[f"custom_{ax}"]: float(fn(v)))
+                        self._keep.append(cb)
+                        setattr(d, f"custom_{ax}", cb)
                 dens = C.pointer(d)
                 self._keep.append(d)
             lib.spec_new(C.byref(self.species[k]), sd["name"].encode(), C.c_float(sd["m_q"]),
@@ -74,12 +84,7 @@ class Em2dRun:
                     self.species, nsp)
         if cfg.get("init_fld"):
             f = A.t_emf_init_fld()
-            f.E_type = A.FLD_TYPE["uniform"] if "E_0" in cfg["init_fld"] else 0
-            f.B_type = A.FLD_TYPE["uniform"] if "B_0" in cfg["init_fld"] else 0
-            if "E_0" in cfg["init_fld"]:
-                f.E_0 = A.float3(*cfg["init_fld"]["E_0"])
-            if "B_0" in cfg["init_fld"]:
-                f.B_0 = A.float3(*cfg["init_fld"]["B_0"])
+            self._field_spec(f, cfg["init_fld"])
             lib.emf_init_fld(C.byref(self.sim.emf), C.byref(f))
         if cfg.get("laser"):
             L = A.t_emf_laser()
@@ -95,14 +100,24 @@ class Em2dRun:
             lib.sim_set_smooth(C.byref(self.sim), C.byref(sm))
         if cfg.get("ext_fld"):
             x = A.t_emf_ext_fld()
-            x.E_type = A.FLD_TYPE["uniform"] if "E_0" in cfg["ext_fld"] else 0
-            x.B_type = A.FLD_TYPE["uniform"] if "B_0" in cfg["ext_fld"] else 0
-            if "E_0" in cfg["ext_fld"]:
-                x.E_0 = A.float3(*cfg["ext_fld"]["E_0"])
-            if "B_0" in cfg["ext_fld"]:
-                x.B_0 = A.float3(*cfg["ext_fld"]["B_0"])
+            self._field_spec(x, cfg["ext_fld"])
             lib.sim_set_ext_fld(C.byref(self.sim), C.byref(x))
         self.n = 0
+
+    def _field_spec(self, f, spec: dict):
+        """Fill an initial/external field struct: uniform "E_0"/"B_0" values or
+        Python callables "E_custom"/"B_custom" (ix, dx, iy, dy) -> 3 floats."""
+        for F in ("E", "B"):
+            if f"{F}_custom" in spec:
+                from zpic_amd import _lib
+
+                fptr, data = A.field_callback(_lib.load(), spec[f"{F}_custom"], self._keep)
+                setattr(f, f"{F}_type", A.FLD_TYPE["custom"])
+                setattr(f, f"{F}_custom", fptr)
+                setattr(f, f"{F}_custom_data", data)
+            elif f"{F}_0" in spec:
+                setattr(f, f"{F}_type", A.FLD_TYPE["uniform"])
+                setattr(f, f"{F}_0", A.float3(*spec[f"{F}_0"]))
 
     def step(self, n: int = 1):
         for _ in range(n):
@@ -144,19 +159,22 @@ class Em2dRun:
 
 
 def weibel(nx=32, ny=32, ppc=2, uz=0.6, uth=0.1, n_sort=16, smooth=None, dx=0.1, dt=0.07,
-           seed=(12345, 67890)) -> dict:
-    """Two counter-streaming electron/positron species (em2d/input/weibel.c:10-42)."""
+           seed=(12345, 67890), m_q2=1.0) -> dict:
+    """Two counter-streaming species (em2d/input/weibel.c:10-42): electrons and,
+    by default, positrons; m_q2=-1 gives two electron populations as in the
+    benchmark configurations."""
     return {"nx": [nx, ny], "box": [nx * dx, ny * dx], "dt": dt, "seed": list(seed),
             "species": [
                 {"name": "electrons", "m_q": -1.0, "ppc": [ppc, ppc], "ufl": [0, 0, uz],
                  "uth": [uth, uth, uth], "n_sort": n_sort},
-                {"name": "positrons", "m_q": 1.0, "ppc": [ppc, ppc], "ufl": [0, 0, -uz],
-                 "uth": [uth, uth, uth], "n_sort": n_sort}],
+                {"name": "positrons" if m_q2 > 0 else "electrons2", "m_q": m_q2, "ppc": [ppc, ppc],
+                 "ufl": [0, 0, -uz], "uth": [uth, uth, uth], "n_sort": n_sort}],
             "smooth": smooth}
 
 
 def lwfa(nx=200, ny=32, dx=0.05, dy=0.2, dt=0.045, start=None, smooth_level=4,
-         a0=2.0, omega0=10.0, W0=4.0, fwhm=2.0, focus=None, laser_start=None) -> dict:
+         a0=2.0, omega0=10.0, W0=4.0, fwhm=2.0, focus=None, laser_start=None,
+         polarization=float(np.pi / 2)) -> dict:
     """Laser wakefield with a moving window (em2d/input/lwfa.c:12-65)."""
     box = [nx * dx, ny * dy]
     focus = box[0] * 0.6 if focus is None else focus
@@ -167,6 +185,6 @@ def lwfa(nx=200, ny=32, dx=0.05, dy=0.2, dt=0.045, start=None, smooth_level=4,
                          "density": {"type": "step", "start": start, "n": 1.0}}],
             "laser": {"type": "gaussian", "start": laser_start, "fwhm": fwhm, "a0": a0,
                       "omega0": omega0, "W0": W0, "focus": focus, "axis": box[1] / 2,
-                      "polarization": float(np.pi / 2)},
+                      "polarization": polarization},
             "moving_window": True,
             "smooth": {"xtype": "compensated", "xlevel": smooth_level}}

@@ -29,6 +29,7 @@ def load() -> C.CDLL:
         "zpic_set_device": (None, [C.c_int]),
         "zpic_set_tiling": (None, [C.c_int, C.c_int, C.c_int]),
         "zpic_synchronize": (None, []),
+        "zpic_rand_clear": (None, []),
         "spec_sync_host": (None, [C.c_void_p]),
         "emf_sync_host": (None, [C.c_void_p]),
         "current_sync_host": (None, [C.c_void_p]),
@@ -38,6 +39,9 @@ def load() -> C.CDLL:
         "spec_track_ids": (None, [C.c_void_p]),
         "spec_get_ids": (None, [C.c_void_p, P(C.c_int64)]),
         "spec_overflow_count": (C.c_int, [C.c_void_p]),
+        "zpic_kernel_timing": (None, [C.c_int]),
+        "zpic_kernel_time": (C.c_double, [C.c_int, P(C.c_longlong)]),
+        "zpic_kernel_timing_reset": (None, []),
     }
     for name, (res, args) in ext.items():
         fn = getattr(lib, name)

@@ -36,7 +36,29 @@ class t_part(C.Structure):
 
 
 DENSITY_FN = C.CFUNCTYPE(C.c_float, C.c_float, C.c_void_p)
+# float3 (*)(int, float, int, float, void*): only C code can implement it
 FIELD_FN = C.CFUNCTYPE(float3, C.c_int, C.c_float, C.c_int, C.c_float, C.c_void_p)
+# what zpic_field_callback calls: writes the 3 components through a pointer
+FIELD_OUT_FN = C.CFUNCTYPE(None, C.c_int, C.c_float, C.c_int, C.c_float, C.POINTER(C.c_float), C.c_void_p)
+
+
+class zpic_field_cb(C.Structure):
+    _fields_ = [("fn", FIELD_OUT_FN), ("user", C.c_void_p)]
+
+
+def field_callback(adapter_lib: C.CDLL, fn, keep: list):
+    """(function pointer, data pointer) for a Python field function
+    fn(ix, dx, iy, dy) -> 3 floats, through the library's
+    zpic_field_callback adapter.  `keep` holds the ctypes objects alive."""
+
+    def body(i, dx, j, dy, out, user):
+        v = fn(i, dx, j, dy)
+        out[0], out[1], out[2] = float(v[0]), float(v[1]), float(v[2])
+
+    cfn = FIELD_OUT_FN(body)
+    cb = zpic_field_cb(cfn, None)
+    keep.extend([cfn, cb])
+    return C.cast(adapter_lib.zpic_field_callback, FIELD_FN), C.cast(C.pointer(cb), C.c_void_p)
 
 
 class t_density(C.Structure):
@@ -70,12 +92,17 @@ class t_emf_laser(C.Structure):
                 ("axis", C.c_float)]
 
 
+_ABI_CACHE: dict = {}
+
+
 def make_abi(with_dev: bool = True) -> SimpleNamespace:
-    """Struct classes for one ABI flavour.
+    """Struct classes for one ABI flavour (one set of classes per flavour).
 
     with_dev=True is this library's layout (trailing `dev` pointer);
     with_dev=False is the reference's layout.
     """
+    if with_dev in _ABI_CACHE:
+        return _ABI_CACHE[with_dev]
     tail = [("dev", C.c_void_p)] if with_dev else []
 
     class t_species(C.Structure):
@@ -107,8 +134,10 @@ def make_abi(with_dev: bool = True) -> SimpleNamespace:
                     ("n_species", C.c_int), ("species", C.POINTER(t_species)),
                     ("emf", t_emf), ("current", t_current), ("moving_window", C.c_int)]
 
-    return SimpleNamespace(t_species=t_species, t_current=t_current, t_emf=t_emf,
-                           t_simulation=t_simulation, with_dev=with_dev)
+    ns = SimpleNamespace(t_species=t_species, t_current=t_current, t_emf=t_emf,
+                         t_simulation=t_simulation, with_dev=with_dev)
+    _ABI_CACHE[with_dev] = ns
+    return ns
 
 
 def bind_api(lib: C.CDLL, abi: SimpleNamespace) -> C.CDLL:
@@ -133,6 +162,10 @@ def bind_api(lib: C.CDLL, abi: SimpleNamespace) -> C.CDLL:
         "emf_advance": (None, [em, cu]),
         "emf_get_energy": (None, [em, P(C.c_double)]),
         "emf_init_fld": (None, [em, P(t_emf_init_fld)]),
+        "emf_set_ext_fld": (None, [em, P(t_emf_ext_fld)]),
+        "emf_report": (None, [em, C.c_byte, C.c_int]),
+        "current_report": (None, [cu, C.c_int]),
+        "spec_report": (None, [sp, C.c_int, P(C.c_int), P(C.c_float * 2)]),
         "emf_time": (C.c_double, []),
         "sim_new": (None, [sim, P(C.c_int), P(C.c_float), C.c_float, C.c_float, C.c_int, sp, C.c_int]),
         "sim_iter": (None, [sim]),

@@ -118,6 +118,7 @@ void current_update(t_current *c)
     a.nrow = c->nrow;
     a.t = d->t;
     a.periodic_x = !c->moving_window;
+    zpic_ktimer_begin(ZPIC_T_CURRENT);
     ZCHECK(zk_current_reduce(&a, zpic_stream()));
 
     /* binomial passes, then the compensator; the y pass count is xlevel,
@@ -137,6 +138,7 @@ void current_update(t_current *c)
             smooth_pass(c, 1, sa, sb);
         }
     }
+    zpic_ktimer_end(ZPIC_T_CURRENT);
     c->iter++;
     d->host_valid = 0;
 }

@@ -56,6 +56,8 @@ void emf_delete(t_emf *emf)
         zpic_dfree(d->Bp);
         zpic_dfree(d->Eext);
         zpic_dfree(d->Bext);
+        free(d->Eext_host);
+        free(d->Bext_host);
         zpic_dfree(d->epartial);
         zpic_dfree(d->energy6);
         free(d);
@@ -257,18 +259,16 @@ void emf_init_fld(t_emf *const emf, t_emf_init_fld *init)
     }
 }
 
-/* Device grid of a custom external field, evaluated once: the callback
- * takes only the node position (em2d/emf.h:37-38). */
-static float *custom_grid(t_emf *emf, float3 (*fn)(int, float, int, float, void *), void *data)
+/* Host grid of a custom external field, evaluated once (the callback takes
+ * only the node position, em2d/emf.h:37-38); uploaded with the fields. */
+static float3 *custom_grid(t_emf *emf, float3 (*fn)(int, float, int, float, void *), void *data)
 {
     const size_t nodes = zpic_grid_nodes(emf->nx);
     float3 *h = malloc(sizeof(float3) * nodes);
+    if (!h) zpic_fatal("out of host memory for an external field grid");
     for (int j = -1; j < emf->nx[1] + 2; j++)
         for (int i = -1; i < emf->nx[0] + 2; i++) h[(i + 1) + (j + 1) * emf->nrow] = fn(i, emf->dx[0], j, emf->dx[1], data);
-    float *g = zpic_dmalloc(sizeof(float3) * nodes);
-    zpic_h2d(g, h, sizeof(float3) * nodes);
-    free(h);
-    return g;
+    return h;
 }
 
 void emf_set_ext_fld(t_emf *const emf, t_emf_ext_fld *ext)
@@ -316,14 +316,14 @@ void emf_set_ext_fld(t_emf *const emf, t_emf_ext_fld *ext)
             emf->E_part = buf + 1 + emf->nrow;
         }
         if (type == EMF_FLD_TYPE_CUSTOM) {
-            float *g = f ? custom_grid(emf, ext->B_custom, ext->B_custom_data)
-                         : custom_grid(emf, ext->E_custom, ext->E_custom_data);
+            float3 *g = f ? custom_grid(emf, ext->B_custom, ext->B_custom_data)
+                          : custom_grid(emf, ext->E_custom, ext->E_custom_data);
             if (f) {
-                zpic_dfree(d->Bext);
-                d->Bext = g;
+                free(d->Bext_host);
+                d->Bext_host = g;
             } else {
-                zpic_dfree(d->Eext);
-                d->Eext = g;
+                free(d->Eext_host);
+                d->Eext_host = g;
             }
         }
     }
@@ -367,6 +367,14 @@ void zpic_emf_ensure(t_emf *emf)
         d->host_dirty = 1;
     }
     if (d->host_dirty) {
+        for (int f = 0; f < 2; f++) {
+            float3 **host = f ? &d->Bext_host : &d->Eext_host;
+            float **dev = f ? &d->Bext : &d->Eext;
+            if (*host) {
+                if (!*dev) *dev = zpic_dmalloc(sizeof(float3) * nodes);
+                zpic_h2d(*dev, *host, sizeof(float3) * nodes);
+            }
+        }
         zpic_h2d(d->E[d->cur], emf->E_buf, sizeof(float3) * nodes);
         zpic_h2d(d->B[d->cur], emf->B_buf, sizeof(float3) * nodes);
         /* the particle fields are re-formed from E and B; the mirror
@@ -418,9 +426,11 @@ void emf_advance(t_emf *emf, const t_current *ccur)
     struct zpic_dev_emf *d = emf->dev;
     zpic_timer_begin(ZPIC_T_EMF);
     const int o = 1 - d->cur;
+    zpic_ktimer_begin(ZPIC_T_FIELD);
     ZCHECK(zk_yee(d->E[d->cur], d->B[d->cur], zpic_current_J(cur), d->E[o], d->B[o], emf->nx[0], emf->nx[1],
                   emf->nrow, emf->dt, emf->dx[0], emf->dx[1], zpic_stream()));
     ZCHECK(zk_field_guards(d->E[o], d->B[o], emf->nx[0], emf->nx[1], emf->nrow, !emf->moving_window, zpic_stream()));
+    zpic_ktimer_end(ZPIC_T_FIELD);
     d->cur = o;
     part_fields(emf);
     emf->iter += 1;

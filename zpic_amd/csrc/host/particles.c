@@ -394,8 +394,10 @@ static void sort_now(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
     const int other = 1 - d->cur;
+    zpic_ktimer_begin(ZPIC_T_SORT);
     ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, &d->t, d->np_upper,
                    zpic_stream()));
+    zpic_ktimer_end(ZPIC_T_SORT);
     d->cur = other;
     d->tail_upper = 0;
     d->moved_since_sort = 0;
@@ -406,13 +408,6 @@ static void upload(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
     const int np = s->np > 0 ? s->np : 0;
-    /* the kernels index grids by cell: refuse particles outside the grid */
-    for (int i = 0; i < np; i++) {
-        const t_part *p = &s->part[i];
-        if (p->ix < 0 || p->ix >= s->nx[0] || p->iy < 0 || p->iy >= s->nx[1])
-            zpic_fatal("species '%s': particle %d has cell (%d, %d) outside the %d x %d grid", s->name, i, p->ix,
-                       p->iy, s->nx[0], s->nx[1]);
-    }
     grow_sets(d, np);
     ensure_stage(d, sizeof(t_part) * (size_t)(np > 0 ? np : 1));
     zpic_h2d(d->stage, s->part, sizeof(t_part) * (size_t)np);
@@ -424,10 +419,23 @@ static void upload(t_species *s)
     d->host_dirty = 0;
 }
 
+/* The kernels index grids by cell: refuse particles outside the grid
+ * before anything reaches the device. */
+static void validate_host_particles(const t_species *s)
+{
+    for (int i = 0; i < s->np; i++) {
+        const t_part *p = &s->part[i];
+        if (p->ix < 0 || p->ix >= s->nx[0] || p->iy < 0 || p->iy >= s->nx[1])
+            zpic_fatal("species '%s': particle %d has cell (%d, %d) outside the %d x %d grid", s->name, i, p->ix,
+                       p->iy, s->nx[0], s->nx[1]);
+    }
+}
+
 void zpic_species_ensure(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
     if (!d) zpic_fatal("species '%s' was not created with spec_new", s->name);
+    if (d->host_dirty) validate_host_particles(s);
     if (!d->ready) {
         zpic_tiles_for(s->nx[0], s->nx[1], &d->t);
         int want = s->np > 0 ? s->np : 1;
@@ -601,7 +609,9 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
         a.epart = d->epart;
     }
     a.tail_blocks = tb;
+    zpic_ktimer_begin(ZPIC_T_PUSH);
     ZCHECK(zk_push(&a, zpic_stream()));
+    zpic_ktimer_end(ZPIC_T_PUSH);
     dc->fresh = 0;
 
     s->iter = next_iter;

@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "random.h"
+#include "zpic_gpu.h"
 
 static uint32_t g_w = 12345u; /* reference default seeds, em2d/random.c:16-17 */
 static uint32_t g_z = 67890u;
@@ -23,6 +24,8 @@ void set_rand_seed(uint32_t w, uint32_t z)
     g_w = w;
     g_z = z;
 }
+
+void zpic_rand_clear(void) { g_have_kept = 0; }
 
 static uint32_t next_u32(void)
 {

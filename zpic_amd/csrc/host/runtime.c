@@ -158,8 +158,10 @@ static struct {
     int used;
     int open;
     double total;
+    long long count;
     int created;
 } g_timer[ZPIC_NTIMERS];
+static int g_ktiming = 0;
 
 static void timer_drain(int w)
 {
@@ -169,6 +171,7 @@ static void timer_drain(int w)
         float ms = 0.0f;
         ZCHECK(hipEventElapsedTime(&ms, g_timer[w].ev[k][0], g_timer[w].ev[k][1]));
         g_timer[w].total += 1e-3 * ms;
+        g_timer[w].count++;
     }
     g_timer[w].used = 0;
 }
@@ -200,4 +203,40 @@ double zpic_timer_total(int w)
 {
     if (g_ready) timer_drain(w);
     return g_timer[w].total;
+}
+
+void zpic_ktimer_begin(int w)
+{
+    if (g_ktiming) zpic_timer_begin(w);
+}
+
+void zpic_ktimer_end(int w)
+{
+    if (g_ktiming) zpic_timer_end(w);
+}
+
+void zpic_kernel_timing(int enable)
+{
+    if (!enable && g_ktiming) {
+        for (int w = ZPIC_T_PUSH; w < ZPIC_NTIMERS; w++) timer_drain(w);
+    }
+    g_ktiming = enable;
+}
+
+double zpic_kernel_time(int kclass, long long *launches)
+{
+    const int w = ZPIC_T_PUSH + kclass;
+    if (w < ZPIC_T_PUSH || w >= ZPIC_NTIMERS) return 0.0;
+    if (g_ready) timer_drain(w);
+    if (launches) *launches = g_timer[w].count;
+    return g_timer[w].total;
+}
+
+void zpic_kernel_timing_reset(void)
+{
+    for (int w = ZPIC_T_PUSH; w < ZPIC_NTIMERS; w++) {
+        if (g_ready) timer_drain(w);
+        g_timer[w].total = 0.0;
+        g_timer[w].count = 0;
+    }
 }

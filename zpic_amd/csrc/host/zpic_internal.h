@@ -62,6 +62,7 @@ struct zpic_dev_emf {
     int cur;
     float *Ep, *Bp;       /* fields seen by particles (external fields on) */
     float *Eext, *Bext;   /* device grids of custom external fields */
+    float3 *Eext_host, *Bext_host; /* the same, evaluated on the host */
     double *epartial, *energy6;
     int nblk;
 };

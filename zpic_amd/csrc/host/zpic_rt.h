@@ -28,8 +28,19 @@ void zpic_sync(void);
 void zpic_tiles_for(int nx, int ny, zk_tiles *t);
 size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */
 
-/* GPU timers for spec_time() / emf_time() */
-enum { ZPIC_T_SPEC = 0, ZPIC_T_EMF = 1, ZPIC_NTIMERS = 2 };
+/* GPU timers: spec_time() / emf_time(), and per-kernel-class timers that
+ * are recorded only while zpic_kernel_timing(1) is on */
+enum {
+    ZPIC_T_SPEC = 0,
+    ZPIC_T_EMF = 1,
+    ZPIC_T_PUSH = 2,
+    ZPIC_T_SORT = 3,
+    ZPIC_T_CURRENT = 4,
+    ZPIC_T_FIELD = 5,
+    ZPIC_NTIMERS = 6
+};
+void zpic_ktimer_begin(int which); /* no-op unless kernel timing is on */
+void zpic_ktimer_end(int which);
 void zpic_timer_begin(int which);
 void zpic_timer_end(int which);
 double zpic_timer_total(int which);

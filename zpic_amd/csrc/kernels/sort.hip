@@ -5,9 +5,10 @@
  * which orders particles by cell to keep memory access local.  Here the key
  * is the tile (TX x TY cells) that the push kernel assigns one workgroup
  * to, so after the sort every tile owns one contiguous range of slots.
- * Removed particles (ZK_DEAD) are dropped.  Order inside a tile follows
- * the atomics and is not stable; the particle order is not part of the
- * em2d contract (the reference itself reorders on sort and on removal).
+ * Removed particles (ZK_DEAD) are dropped.  Inside a tile the particles
+ * are interleaved across cells (see interleave()); the order is not stable
+ * and is not part of the em2d contract (the reference itself reorders on
+ * sort and on removal).
  *
  * Three launches: count (per-tile histogram), scan (tile offsets, one
  * workgroup), scatter (write every live particle to its tile range).  Count
@@ -74,7 +75,24 @@ __global__ void snapshot_kernel(const zk_spec_ctr *ctr, int *n_old)
     *n_old = ctr->np_total;
 }
 
-__global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, int *cursor, zk_tiles t)
+/* Slot of the rank-th particle of a tile: consecutive ranks (which come
+ * from one cell, as the input is cell-ordered within a tile) are spread by
+ * a prime stride over the tile's range, so the 64 lanes of a wave push
+ * particles of different cells and their LDS atomics do not collide. */
+__device__ inline int interleave(int rank, int count)
+{
+    const int primes[6] = {97, 101, 103, 107, 109, 113};
+    int p = primes[0];
+    for (int k = 0; k < 6; k++)
+        if (count % primes[k]) {
+            p = primes[k];
+            break;
+        }
+    return (int)(((long long)rank * p) % count);
+}
+
+__global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, int *cursor, const int *off,
+                                 zk_tiles t)
 {
     const int n = *n_old;
     const int stride = gridDim.x * blockDim.x;
@@ -86,8 +104,10 @@ __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, i
             iy = src.iy[i];
             key = tile_key(ix, iy, t);
         }
-        const int d = zk_run_reserve(key, cursor);
+        int d = zk_run_reserve(key, cursor);
         if (key >= 0) {
+            const int base = off[key];
+            d = base + interleave(d - base, off[key + 1] - base);
             dst.ix[d] = ix;
             dst.iy[d] = iy;
             dst.x[d] = src.x[i];
@@ -163,7 +183,7 @@ extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ct
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, n_old);
     hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, ctr, cnt, *t);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles);
-    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, *t);
+    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t);
     ZK_LAUNCH_CHECK();
 }
 
