@@ -69,6 +69,25 @@ def cpu_baseline(sample_nx: int, ppc: int, steps: int, nproc: int) -> dict | Non
                        f"{nproc} independent processes on {nproc} host cores, {wall:.1f} s wall")}
 
 
+def copy_probe(nbytes: int) -> int:
+    """One hipMemcpy device-to-device of nbytes: the runtime's copy kernel
+    reads and writes exactly nbytes, the yardstick for HBM byte counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,11 +96,19 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ppc", type=int, default=4)
     ap.add_argument("--n-sort", type=int, default=20)
+    ap.add_argument("--tile", type=int, default=0, help="tile width in cells (0: library default)")
+    ap.add_argument("--halo", type=int, default=0, help="tile drift margin in cells (0: library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-nx", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES through the HIP runtime "
+                         "(calibrates the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -95,6 +122,8 @@ def main() -> int:
     from zpic_amd import em2d
 
     lib = _lib.load()
+    if args.tile or args.halo:
+        lib.zpic_set_tiling(args.tile, args.tile, args.halo)
     nx = args.nx
     ppc = args.ppc
     dx, dt, uz, uth = 0.1, 0.07, 0.6, 0.1

@@ -72,6 +72,19 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm((a - b).ravel()) / (n if n > 0 else 1.0))
 
 
+def field_err(sa: dict, sb: dict, f: str) -> float:
+    """Relative L2 difference of grid f between two states.  For B the
+    reference norm is max(|B|, |E|): early in a run B is a small, noisy
+    curl of E and its own norm would turn FP32 rounding of E into large
+    relative numbers."""
+    if f != "B":
+        return rel_l2(sa[f], sb[f])
+    b = np.asarray(sb["B"], dtype=np.float64)
+    e = np.asarray(sb["E"], dtype=np.float64)
+    d = np.linalg.norm((np.asarray(sa["B"], dtype=np.float64) - b).ravel())
+    return float(d / max(np.linalg.norm(b.ravel()), np.linalg.norm(e.ravel()), 1e-300))
+
+
 def oracle_ids(lib, run, k: int) -> np.ndarray:
     n = run.species[k].np
     return np.ctypeslib.as_array(lib.orc_ids(C.byref(run.species[k])), shape=(n,)).copy()

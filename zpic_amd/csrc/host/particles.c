@@ -390,13 +390,13 @@ static void grow_sets(struct zpic_dev_species *d, int need)
     d->cap = cap;
 }
 
-static void sort_now(t_species *s)
+static void sort_now(t_species *s, int spread)
 {
     struct zpic_dev_species *d = s->dev;
     const int other = 1 - d->cur;
     zpic_ktimer_begin(ZPIC_T_SORT);
     ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, &d->t, d->np_upper,
-                   zpic_stream()));
+                   spread, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_SORT);
     d->cur = other;
     d->tail_upper = 0;
@@ -415,7 +415,7 @@ static void upload(t_species *s)
     if (d->track_ids && np > 0) zpic_h2d(d->set[d->cur].id, d->host_ids, sizeof(int64_t) * (size_t)np);
     d->np_upper = np;
     d->tail_upper = np;
-    sort_now(s);
+    sort_now(s, 1);
     d->host_dirty = 0;
 }
 
@@ -477,7 +477,7 @@ void spec_sync_host(t_species *s)
     zk_spec_ctr c;
     zpic_d2h(&c, d->ctr, sizeof(c));
     if (c.n_dead > 0 || c.n_sorted != c.np_total) {
-        sort_now(s);
+        sort_now(s, 0);
         zpic_d2h(&c, d->ctr, sizeof(c));
     }
     const int np = c.np_total;
@@ -623,7 +623,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     const float qm = s->q * s->m_q;
     ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0], (double)s->dx[1],
                             zpic_stream()));
-    if ((s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0) sort_now(s);
+    if ((s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0) sort_now(s, 0);
     zpic_timer_end(ZPIC_T_SPEC);
     d->host_valid = 0;
 }

@@ -201,6 +201,37 @@ struct Consts {
     float tem, dt_dx, dt_dy, qnx, qny, q;
 };
 
+struct Part {
+    int ix, iy;
+    float x, y, ux, uy, uz;
+};
+
+/* Particle streams are read and written once per step: nontemporal
+ * accesses keep them from evicting the field windows from the caches. */
+__device__ inline Part load_part(const zk_parts &P, int i)
+{
+    Part c;
+    c.ix = __builtin_nontemporal_load(&P.ix[i]);
+    c.iy = __builtin_nontemporal_load(&P.iy[i]);
+    c.x = __builtin_nontemporal_load(&P.x[i]);
+    c.y = __builtin_nontemporal_load(&P.y[i]);
+    c.ux = __builtin_nontemporal_load(&P.ux[i]);
+    c.uy = __builtin_nontemporal_load(&P.uy[i]);
+    c.uz = __builtin_nontemporal_load(&P.uz[i]);
+    return c;
+}
+
+__device__ inline void store_part(const zk_parts &P, int i, const Part &c)
+{
+    __builtin_nontemporal_store(c.ix, &P.ix[i]);
+    __builtin_nontemporal_store(c.iy, &P.iy[i]);
+    __builtin_nontemporal_store(c.x, &P.x[i]);
+    __builtin_nontemporal_store(c.y, &P.y[i]);
+    __builtin_nontemporal_store(c.ux, &P.ux[i]);
+    __builtin_nontemporal_store(c.uy, &P.uy[i]);
+    __builtin_nontemporal_store(c.uz, &P.uz[i]);
+}
+
 /* Push one particle whose cell is (ci, cj) in the sources' index space;
  * returns its time-centred kinetic energy term (em2d/particles.c:1125-1228). */
 template <class F, class D>
@@ -307,28 +338,27 @@ __global__ void __launch_bounds__(256) push_kernel(zk_push_args a)
         const LdsFields lf = {fl, t.WX, n};
         const LdsCurrent lj = {jl, t.WX, n};
         const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
-        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            int ix = P.ix[i], iy = P.iy[i];
-            if (ix == ZK_DEAD) continue;
-            float x = P.x[i], y = P.y[i], ux = P.ux[i], uy = P.uy[i], uz = P.uz[i];
-            const int lx = ix - tx0, ly = iy - ty0;
+        /* the next particle's seven values are loaded before the current one
+         * is pushed, so the HBM latency of the stream overlaps the arithmetic */
+        int i = i0 + threadIdx.x;
+        Part nxt;
+        if (i < i1) nxt = load_part(P, i);
+        for (; i < i1; i += blockDim.x) {
+            Part c = nxt;
+            if (i + (int)blockDim.x < i1) nxt = load_part(P, i + blockDim.x);
+            if (c.ix == ZK_DEAD) continue;
+            const int lx = c.ix - tx0, ly = c.iy - ty0;
             int di, dj;
             if (lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H) {
-                kin += push_one(lf, lj, ix - ox, iy - oy, x, y, ux, uy, uz, di, dj, k);
+                kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
             } else {
                 nov++;
-                kin += push_one(gf, gj, ix + 1, iy + 1, x, y, ux, uy, uz, di, dj, k);
+                kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
             }
-            ix += di;
-            iy += dj;
-            if (boundaries(ix, iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-            P.ix[i] = ix;
-            P.iy[i] = iy;
-            P.x[i] = x;
-            P.y[i] = y;
-            P.ux[i] = ux;
-            P.uy[i] = uy;
-            P.uz[i] = uz;
+            c.ix += di;
+            c.iy += dj;
+            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+            store_part(P, i, c);
         }
         __syncthreads();
         /* store the current window of this tile */
@@ -343,22 +373,15 @@ __global__ void __launch_bounds__(256) push_kernel(zk_push_args a)
         const int i0 = a.ctr->n_sorted, i1 = a.ctr->np_total;
         const int stride = a.tail_blocks * blockDim.x;
         for (int i = i0 + ((int)blockIdx.x - t.ntiles) * blockDim.x + threadIdx.x; i < i1; i += stride) {
-            int ix = P.ix[i], iy = P.iy[i];
-            if (ix == ZK_DEAD) continue;
-            float x = P.x[i], y = P.y[i], ux = P.ux[i], uy = P.uy[i], uz = P.uz[i];
+            Part c = load_part(P, i);
+            if (c.ix == ZK_DEAD) continue;
             int di, dj;
             nov++;
-            kin += push_one(gf, gj, ix + 1, iy + 1, x, y, ux, uy, uz, di, dj, k);
-            ix += di;
-            iy += dj;
-            if (boundaries(ix, iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-            P.ix[i] = ix;
-            P.iy[i] = iy;
-            P.x[i] = x;
-            P.y[i] = y;
-            P.ux[i] = ux;
-            P.uy[i] = uy;
-            P.uz[i] = uz;
+            kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+            c.ix += di;
+            c.iy += dj;
+            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+            store_part(P, i, c);
         }
     }
     /* per-block energy partial (summed in fixed order by zk_spec_finalize) */
@@ -407,6 +430,12 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
     const size_t lds = (size_t)9 * n * sizeof(float);
     const int blocks = a->t.ntiles + a->tail_blocks;
     if (blocks <= 0) return 0;
+    if (lds > 65536) {
+        /* windows of large tiles need more than the default 64 KiB of LDS */
+        const int e = (int)hipFuncSetAttribute((const void *)push_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds);
+        if (e) return e;
+    }
     hipLaunchKernelGGL(push_kernel, dim3(blocks), dim3(256), lds, s, *a);
     ZK_LAUNCH_CHECK();
 }

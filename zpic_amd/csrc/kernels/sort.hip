@@ -5,10 +5,13 @@
  * which orders particles by cell to keep memory access local.  Here the key
  * is the tile (TX x TY cells) that the push kernel assigns one workgroup
  * to, so after the sort every tile owns one contiguous range of slots.
- * Removed particles (ZK_DEAD) are dropped.  Inside a tile the particles
- * are interleaved across cells (see interleave()); the order is not stable
- * and is not part of the em2d contract (the reference itself reorders on
- * sort and on removal).
+ * Removed particles (ZK_DEAD) are dropped.  With `spread` (used when a
+ * cell-ordered host buffer is uploaded) the particles of a tile are
+ * interleaved across cells (see interleave()); later sorts keep runs
+ * contiguous, which preserves that interleaving for particles that stay
+ * and keeps the writes coalesced.  The order is not stable and is not part
+ * of the em2d contract (the reference itself reorders on sort and on
+ * removal).
  *
  * Three launches: count (per-tile histogram), scan (tile offsets, one
  * workgroup), scatter (write every live particle to its tile range).  Count
@@ -92,7 +95,7 @@ __device__ inline int interleave(int rank, int count)
 }
 
 __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, int *cursor, const int *off,
-                                 zk_tiles t)
+                                 zk_tiles t, int spread)
 {
     const int n = *n_old;
     const int stride = gridDim.x * blockDim.x;
@@ -106,8 +109,10 @@ __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, i
         }
         int d = zk_run_reserve(key, cursor);
         if (key >= 0) {
-            const int base = off[key];
-            d = base + interleave(d - base, off[key + 1] - base);
+            if (spread) {
+                const int base = off[key];
+                d = base + interleave(d - base, off[key + 1] - base);
+            }
             dst.ix[d] = ix;
             dst.iy[d] = iy;
             dst.x[d] = src.x[i];
@@ -175,7 +180,7 @@ unsigned grid_for(long long n, int bs, unsigned cap)
 } // namespace
 
 extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *cnt, int *off,
-                              int *cursor, const zk_tiles *t, int np_upper, zk_stream_t s)
+                              int *cursor, const zk_tiles *t, int np_upper, int spread, zk_stream_t s)
 {
     const unsigned g = grid_for(np_upper, 256, 8192);
     /* the slot count in use before the scan is kept in off[ntiles + 1] */
@@ -183,7 +188,7 @@ extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ct
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, n_old);
     hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, ctr, cnt, *t);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles);
-    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t);
+    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t, spread);
     ZK_LAUNCH_CHECK();
 }
 

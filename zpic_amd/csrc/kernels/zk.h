@@ -80,7 +80,7 @@ int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int nparts,
 /* tile counting sort: src -> dst; off has ntiles + 2 entries, cnt and cursor ntiles */
 int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr,
                    int *cnt, int *off, int *cursor, const zk_tiles *t,
-                   int np_upper, zk_stream_t s);
+                   int np_upper, int spread, zk_stream_t s);
 /* AoS staging <-> SoA */
 int zk_unpack(const void *aos, int n, const zk_parts *dst, int64_t id0,
                      zk_spec_ctr *ctr, int append, zk_stream_t s);
