@@ -29,7 +29,7 @@ extern "C" {
 void zpic_set_device(int device);
 
 /* Tile shape for species/currents created afterwards: tx x ty cells with a
- * drift margin of `halo` cells; 0 keeps the default (16, 16, 2).  Tiles
+ * drift margin of `halo` cells; 0 keeps the default (16, 16, 3).  Tiles
  * must divide the grid; a side that no preferred tile width divides uses
  * the largest divisor <= 64 that leaves room for the margin. */
 void zpic_set_tiling(int tx, int ty, int halo);
@@ -39,6 +39,10 @@ void zpic_set_tiling(int tx, int ty, int halo);
  * 69-70); clearing it lets independent runs in one process start from
  * identical generator states. */
 void zpic_rand_clear(void);
+
+/* Occupancy variant of the push kernel: 0 (default) keeps every value in
+ * registers, 8 requests 8 waves per SIMD with a few register spills. */
+void zpic_set_push_waves(int waves);
 
 /* Block until all queued GPU work is done. */
 void zpic_synchronize(void);

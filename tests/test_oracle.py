@@ -119,7 +119,7 @@ def test_gauss_law_defect_is_constant(oracle):
         g1 = gauss_defect(run.state()["E"], _rho(run, 32, 32), dx, dy)
         run.step(30)
         g2 = gauss_defect(run.state()["E"], _rho(run, 32, 32), dx, dy)
-        assert np.abs(g2 - g1).max() <= 1e-4 * np.abs(g1).max()
+        assert np.abs(g2 - g1).max() <= 1e-5 * np.abs(run.charge(0)).max()
     finally:
         run.close()
 

@@ -276,6 +276,8 @@ static void species_free_device(struct zpic_dev_species *d)
         zpic_dfree(p->id);
         memset(p, 0, sizeof(*p));
     }
+    zpic_dfree(d->ovf_list);
+    d->ovf_list = NULL;
     zpic_dfree(d->tile_off);
     zpic_dfree(d->cnt);
     zpic_dfree(d->cursor);
@@ -387,6 +389,8 @@ static void grow_sets(struct zpic_dev_species *d, int need)
         zpic_dfree(old.id);
         d->set[k] = nw;
     }
+    zpic_dfree(d->ovf_list);
+    d->ovf_list = zpic_dmalloc(sizeof(int) * (size_t)cap);
     d->cap = cap;
 }
 
@@ -454,7 +458,7 @@ void zpic_species_ensure(t_species *s)
         d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         d->cursor = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         ZCHECK(zk_fill_f32((float *)d->cnt, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */
-        d->epart_cap = d->t.ntiles + 1024;
+        d->epart_cap = d->t.ntiles + ZPIC_GLOBAL_BLOCKS;
         d->epart = zpic_dmalloc(sizeof(double) * (size_t)d->epart_cap);
         d->ready = 1;
         d->host_dirty = 1;
@@ -599,16 +603,11 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     const int next_iter = s->iter + 1;
     const int move = s->moving_window && ((next_iter * s->dt) > (s->dx[0] * (s->n_move + 1)));
     a.shift = move;
-    int tb = d->tail_upper > 0 ? (d->tail_upper + 255) / 256 : 0;
-    if (tb > 1024) tb = 1024;
-    if (d->t.ntiles + tb > d->epart_cap) {
-        zpic_sync();
-        zpic_dfree(d->epart);
-        d->epart_cap = d->t.ntiles + tb + 1024;
-        d->epart = zpic_dmalloc(sizeof(double) * (size_t)d->epart_cap);
-        a.epart = d->epart;
-    }
-    a.tail_blocks = tb;
+    /* the global pass takes the listed out-of-window particles and the tail */
+    const int tb = ZPIC_GLOBAL_BLOCKS;
+    a.ovf_list = d->ovf_list;
+    a.global_blocks = tb;
+    a.waves = zpic_push_waves();
     zpic_ktimer_begin(ZPIC_T_PUSH);
     ZCHECK(zk_push(&a, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_PUSH);
@@ -621,7 +620,8 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
         d->moved_since_sort++;
     }
     const float qm = s->q * s->m_q;
-    ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0], (double)s->dx[1],
+    ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + ZPIC_GLOBAL_BLOCKS, (double)qm, (double)s->dx[0],
+                            (double)s->dx[1],
                             zpic_stream()));
     if ((s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0) sort_now(s, 0);
     zpic_timer_end(ZPIC_T_SPEC);

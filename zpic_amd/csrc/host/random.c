@@ -7,6 +7,10 @@
  * polar Box-Muller method, which returns one deviate and keeps the other
  * for the next call.  The kept deviate survives set_rand_seed, as in the
  * reference, so a run reproduces the reference's injected momenta exactly.
+ *
+ * The generator state is per thread (the reference's is process-global):
+ * a single-threaded program sees the reference's behaviour, and several
+ * host threads can initialise independent simulations at once.
  */
 #include <math.h>
 #include <stdint.h>
@@ -14,10 +18,10 @@
 #include "random.h"
 #include "zpic_gpu.h"
 
-static uint32_t g_w = 12345u; /* reference default seeds, em2d/random.c:16-17 */
-static uint32_t g_z = 67890u;
-static double g_kept;
-static int g_have_kept = 0;
+static _Thread_local uint32_t g_w = 12345u; /* reference default seeds, em2d/random.c:16-17 */
+static _Thread_local uint32_t g_z = 67890u;
+static _Thread_local double g_kept;
+static _Thread_local int g_have_kept = 0;
 
 void set_rand_seed(uint32_t w, uint32_t z)
 {

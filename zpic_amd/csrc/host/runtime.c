@@ -21,7 +21,11 @@ void zpic_set_device(int device);
 void zpic_set_tiling(int tx, int ty, int halo);
 void zpic_synchronize(void);
 
-static int g_tx = 16, g_ty = 16, g_halo = 2;
+static int g_tx = 16, g_ty = 16, g_halo = 3;
+static int g_push_waves = 0;
+
+void zpic_set_push_waves(int waves) { g_push_waves = (waves == 8) ? 8 : 0; }
+int zpic_push_waves(void) { return g_push_waves; }
 
 void zpic_check(int status, const char *what, const char *file, int line)
 {
@@ -53,7 +57,7 @@ void zpic_set_tiling(int tx, int ty, int halo)
 {
     g_tx = tx > 0 ? tx : 16;
     g_ty = ty > 0 ? ty : 16;
-    g_halo = halo > 0 ? halo : 2;
+    g_halo = halo > 0 ? halo : 3;
 }
 
 static void runtime_init(void)

@@ -17,6 +17,9 @@
 #include "zpic_gpu.h"
 #include "zpic_rt.h"
 
+/* blocks of the global (out-of-window and tail) push pass */
+#define ZPIC_GLOBAL_BLOCKS 512
+
 struct zpic_dev_species {
     int ready;            /* device buffers allocated and filled */
     int host_dirty;       /* host mirror edited: upload before the next push */
@@ -29,6 +32,7 @@ struct zpic_dev_species {
     zk_spec_ctr *ctr;     /* device counters */
     zk_tiles t;
     int *tile_off, *cnt, *cursor;
+    int *ovf_list;        /* slots listed by the tile pass for the global pass */
     double *epart;
     int epart_cap;
     void *stage;          /* device AoS staging for uploads/downloads */

@@ -26,6 +26,7 @@ void zpic_d2h(void *dst, const void *src, size_t bytes);      /* synchronous */
 void zpic_d2d_async(void *dst, const void *src, size_t bytes); /* on the stream */
 void zpic_sync(void);
 void zpic_tiles_for(int nx, int ny, zk_tiles *t);
+int zpic_push_waves(void);
 size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */
 
 /* GPU timers: spec_time() / emf_time(), and per-kernel-class timers that

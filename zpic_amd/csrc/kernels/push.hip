@@ -19,11 +19,11 @@
  * windows are summed into J afterwards by current.hip in a fixed order.
  *
  * A particle that has drifted more than H cells out of its tile since the
- * last sort takes the global path: gathers from the guard-inclusive field
- * buffers exactly as the reference indexes them, and deposits with atomics
- * into the raw overflow grid `Jov`.  Blocks past the last tile run the same
- * global path over the unsorted tail (particles injected by the moving
- * window since the last sort).
+ * last sort is listed by the tile pass and pushed by a second kernel on the
+ * global path: gathers from the guard-inclusive field buffers exactly as
+ * the reference indexes them, and deposits with atomics into the raw
+ * overflow grid `Jov`.  The same kernel pushes the unsorted tail
+ * (particles injected by the moving window since the last sort).
  */
 #include "common.h"
 
@@ -297,9 +297,94 @@ __device__ inline bool boundaries(int &ix, int &iy, int nx, int ny, int mw, int 
     return false;
 }
 
-__global__ void __launch_bounds__(256) push_kernel(zk_push_args a)
+/* Tile pass: one workgroup per tile, gathers and deposits through LDS.
+ * Particles outside the tile window are only listed here and pushed by
+ * push_global_kernel, which keeps this kernel's register budget to the
+ * common path. */
+template <int MIN_WAVES>
+__global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args a)
 {
     extern __shared__ float sm[];
+    __shared__ double red[16];
+    __shared__ int ndead_blk;
+    const zk_tiles &t = a.t;
+    const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
+    zk_parts P = a.p;
+    double kin = 0.0;
+    int ndead = 0;
+    if (threadIdx.x == 0) ndead_blk = 0;
+
+    const int tile = zk_xcd_item(blockIdx.x, t.ntiles);
+    const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
+    const int n = t.WX * t.WY;
+    const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H; /* cell index of window node 0 */
+    float *fl = sm;
+    float *jl = sm + 6 * n;
+    /* stage the window of E and B; nodes outside the guard-inclusive
+     * buffer are never read by an in-window particle */
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        const int gx = ox + m % t.WX, gy = oy + m / t.WX;
+        const bool in = gx >= -1 && gx <= t.nx + 1 && gy >= -1 && gy <= t.ny + 1;
+        const int g = 3 * ((gx + 1) + (gy + 1) * a.nrow);
+        fl[0 * n + m] = in ? a.E[g + 0] : 0.0f;
+        fl[1 * n + m] = in ? a.E[g + 1] : 0.0f;
+        fl[2 * n + m] = in ? a.E[g + 2] : 0.0f;
+        fl[3 * n + m] = in ? a.B[g + 0] : 0.0f;
+        fl[4 * n + m] = in ? a.B[g + 1] : 0.0f;
+        fl[5 * n + m] = in ? a.B[g + 2] : 0.0f;
+        jl[0 * n + m] = 0.0f;
+        jl[1 * n + m] = 0.0f;
+        jl[2 * n + m] = 0.0f;
+    }
+    __syncthreads();
+    const LdsFields lf = {fl, t.WX, n};
+    const LdsCurrent lj = {jl, t.WX, n};
+    const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
+    const int lane = __lane_id();
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        Part c = load_part(P, i);
+        const int lx = c.ix - tx0, ly = c.iy - ty0;
+        const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
+        const bool live = c.ix != ZK_DEAD;
+        /* list the live particles outside the window, one atomic per wave */
+        const unsigned long long out = __ballot(live && !inside);
+        if (out) {
+            int base = 0;
+            const int leader = __ffsll((long long)out) - 1;
+            if (lane == leader) base = atomicAdd(&a.ctr->n_listed, __popcll(out));
+            base = __shfl(base, leader, ZK_WAVE);
+            if (live && !inside) a.ovf_list[base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+        }
+        if (!live || !inside) continue;
+        int di, dj;
+        kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+        c.ix += di;
+        c.iy += dj;
+        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+        store_part(P, i, c);
+    }
+    __syncthreads();
+    /* store the current window of this tile */
+    float *dst = a.region + (size_t)tile * 3 * n;
+    if (a.fresh) {
+        for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] = jl[m];
+    } else {
+        for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] += jl[m];
+    }
+    /* per-tile energy partial (summed in fixed order by zk_spec_finalize) */
+    const double s = zk_block_sum(kin, red);
+    if (threadIdx.x == 0) a.epart[tile] = s;
+    if (ndead) atomicAdd(&ndead_blk, ndead);
+    __syncthreads();
+    if (threadIdx.x == 0 && ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
+}
+
+/* Global pass: the particles listed by the tile pass, then the unsorted
+ * tail (injected since the last sort), with gathers from the guard-inclusive
+ * field grids as the reference indexes them and atomics into the raw
+ * overflow current. */
+__global__ void __launch_bounds__(256) push_global_kernel(zk_push_args a)
+{
     __shared__ double red[16];
     __shared__ int ndead_blk, nov_blk;
     const zk_tiles &t = a.t;
@@ -310,83 +395,23 @@ __global__ void __launch_bounds__(256) push_kernel(zk_push_args a)
     double kin = 0.0;
     int ndead = 0, nov = 0;
     if (threadIdx.x == 0) ndead_blk = nov_blk = 0;
-
-    if ((int)blockIdx.x < t.ntiles) {
-        const int tile = zk_xcd_item(blockIdx.x, t.ntiles);
-        const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
-        const int n = t.WX * t.WY;
-        const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H; /* cell index of window node 0 */
-        float *fl = sm;
-        float *jl = sm + 6 * n;
-        /* stage the window of E and B; nodes outside the guard-inclusive
-         * buffer are never read by an in-window particle */
-        for (int m = threadIdx.x; m < n; m += blockDim.x) {
-            const int gx = ox + m % t.WX, gy = oy + m / t.WX;
-            const bool in = gx >= -1 && gx <= t.nx + 1 && gy >= -1 && gy <= t.ny + 1;
-            const int g = 3 * ((gx + 1) + (gy + 1) * a.nrow);
-            fl[0 * n + m] = in ? a.E[g + 0] : 0.0f;
-            fl[1 * n + m] = in ? a.E[g + 1] : 0.0f;
-            fl[2 * n + m] = in ? a.E[g + 2] : 0.0f;
-            fl[3 * n + m] = in ? a.B[g + 0] : 0.0f;
-            fl[4 * n + m] = in ? a.B[g + 1] : 0.0f;
-            fl[5 * n + m] = in ? a.B[g + 2] : 0.0f;
-            jl[0 * n + m] = 0.0f;
-            jl[1 * n + m] = 0.0f;
-            jl[2 * n + m] = 0.0f;
-        }
-        __syncthreads();
-        const LdsFields lf = {fl, t.WX, n};
-        const LdsCurrent lj = {jl, t.WX, n};
-        const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
-        /* the next particle's seven values are loaded before the current one
-         * is pushed, so the HBM latency of the stream overlaps the arithmetic */
-        int i = i0 + threadIdx.x;
-        Part nxt;
-        if (i < i1) nxt = load_part(P, i);
-        for (; i < i1; i += blockDim.x) {
-            Part c = nxt;
-            if (i + (int)blockDim.x < i1) nxt = load_part(P, i + blockDim.x);
-            if (c.ix == ZK_DEAD) continue;
-            const int lx = c.ix - tx0, ly = c.iy - ty0;
-            int di, dj;
-            if (lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H) {
-                kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-            } else {
-                nov++;
-                kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-            }
-            c.ix += di;
-            c.iy += dj;
-            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-            store_part(P, i, c);
-        }
-        __syncthreads();
-        /* store the current window of this tile */
-        float *dst = a.region + (size_t)tile * 3 * n;
-        if (a.fresh) {
-            for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] = jl[m];
-        } else {
-            for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] += jl[m];
-        }
-    } else {
-        /* unsorted tail: global path for every particle */
-        const int i0 = a.ctr->n_sorted, i1 = a.ctr->np_total;
-        const int stride = a.tail_blocks * blockDim.x;
-        for (int i = i0 + ((int)blockIdx.x - t.ntiles) * blockDim.x + threadIdx.x; i < i1; i += stride) {
-            Part c = load_part(P, i);
-            if (c.ix == ZK_DEAD) continue;
-            int di, dj;
-            nov++;
-            kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-            c.ix += di;
-            c.iy += dj;
-            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-            store_part(P, i, c);
-        }
+    const int listed = a.ctr->n_listed;
+    const int tail0 = a.ctr->n_sorted;
+    const int total = listed + (a.ctr->np_total - tail0);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const int i = q < listed ? a.ovf_list[q] : tail0 + (q - listed);
+        Part c = load_part(P, i);
+        if (c.ix == ZK_DEAD) continue;
+        int di, dj;
+        nov++;
+        kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+        c.ix += di;
+        c.iy += dj;
+        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+        store_part(P, i, c);
     }
-    /* per-block energy partial (summed in fixed order by zk_spec_finalize) */
     const double s = zk_block_sum(kin, red);
-    if (threadIdx.x == 0) a.epart[blockIdx.x] = s;
+    if (threadIdx.x == 0) a.epart[t.ntiles + blockIdx.x] = s;
     if (ndead) atomicAdd(&ndead_blk, ndead);
     if (nov) atomicAdd(&nov_blk, nov);
     __syncthreads();
@@ -413,6 +438,7 @@ __global__ void finalize_kernel(zk_spec_ctr *ctr, const double *epart, int npart
         for (int i = 0; i < (int)blockDim.x; i++) tot += part[i];
         ctr->energy = qm * tot * dx0 * dx1;
         ctr->npush += (unsigned long long)(ctr->np_total - ctr->n_dead);
+        ctr->n_listed = 0;
     }
 }
 
@@ -428,15 +454,22 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
 {
     const int n = a->t.WX * a->t.WY;
     const size_t lds = (size_t)9 * n * sizeof(float);
-    const int blocks = a->t.ntiles + a->tail_blocks;
-    if (blocks <= 0) return 0;
+    /* occupancy variant: 0 lets the compiler keep every value in registers
+     * (about 6 waves per SIMD); 8 asks for 8 waves per SIMD at the price of
+     * a few register spills */
+    const void *fn = a->waves == 8 ? (const void *)push_tile_kernel<8> : (const void *)push_tile_kernel<1>;
     if (lds > 65536) {
         /* windows of large tiles need more than the default 64 KiB of LDS */
-        const int e = (int)hipFuncSetAttribute((const void *)push_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)lds);
+        const int e = (int)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e) return e;
     }
-    hipLaunchKernelGGL(push_kernel, dim3(blocks), dim3(256), lds, s, *a);
+    if (a->t.ntiles > 0) {
+        if (a->waves == 8)
+            hipLaunchKernelGGL(push_tile_kernel<8>, dim3(a->t.ntiles), dim3(256), lds, s, *a);
+        else
+            hipLaunchKernelGGL(push_tile_kernel<1>, dim3(a->t.ntiles), dim3(256), lds, s, *a);
+    }
+    if (a->global_blocks > 0) hipLaunchKernelGGL(push_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
     ZK_LAUNCH_CHECK();
 }
 

@@ -35,7 +35,8 @@ typedef struct zk_spec_ctr {
     int np_total;               /* slots in use: tile ranges + unsorted tail */
     int n_sorted;               /* end of the tile ranges                     */
     int n_dead;                 /* removed particles still occupying slots    */
-    int n_overflow;             /* pushes that left the tile window this step */
+    int n_overflow;             /* pushes on the global path, accumulated     */
+    int n_listed;               /* out-of-window particles listed this step   */
     unsigned long long npush;   /* accumulated particle pushes                */
     double energy;              /* kinetic energy of the last push            */
 } zk_spec_ctr;
@@ -68,7 +69,9 @@ typedef struct zk_push_args {
     zk_tiles t;
     int moving_window;          /* x absorbing + window shift           */
     int shift;                  /* 1 when the window moves this step    */
-    int tail_blocks;            /* extra blocks for the unsorted tail    */
+    int *ovf_list;              /* slots of out-of-window particles     */
+    int global_blocks;          /* blocks of the global (overflow) pass */
+    int waves;                  /* tile-pass occupancy variant: 0 or 8   */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
 
