@@ -86,6 +86,26 @@ typedef struct zpic_field_cb {
 } zpic_field_cb;
 float3 zpic_field_callback(int ix, float dx, int iy, float dy, void *data);
 
+/* Batches of independent simulations (parameter scans).  No counterpart
+ * in the reference, whose nearest relative runs the simulations of a scan
+ * one after another (mods/em1d_param_scan/main.c:50-98).
+ *
+ * zpic_batch_new takes n simulations made with sim_new (any grid size,
+ * species count, particle count, filter and sort period; periodic, no
+ * moving window, no external fields) and zpic_batch_run advances every
+ * one of them by `steps` iterations: each iteration is exactly sim_iter
+ * of each simulation, but all simulations advance together, one kernel
+ * launch per phase of the step, replayed from a captured HIP graph.
+ * Afterwards every simulation is in the state sim_iter would leave it in
+ * (host mirrors refreshed by the usual *_sync_host calls).  A simulation
+ * may be stepped with sim_iter, or edited, between batch runs; its grid,
+ * species and filter must not change while the batch exists.  Free the
+ * batch before the simulations. */
+typedef struct zpic_batch zpic_batch;
+zpic_batch *zpic_batch_new(t_simulation *const *sims, int n);
+void zpic_batch_run(zpic_batch *batch, int steps);
+void zpic_batch_delete(zpic_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,6 +139,18 @@ def _child(code: str) -> subprocess.CompletedProcess:
         r = Em2dRun(lib, weibel(8, 8, 1))
         r.species[0].part[3].ix = 99
         lib.spec_advance(C.byref(r.species[0]), C.byref(r.sim.emf), C.byref(r.sim.current))""", "outside the 8 x 8 grid"),
+    ("batch_moving_window", """
+        r = Em2dRun(lib, dict(weibel(8, 8, 1), moving_window=True))
+        lib.zpic_batch_new.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+        lib.zpic_batch_new((C.c_void_p * 1)(C.addressof(r.sim)), 1)""", "uses a moving window"),
+    ("batch_ext_fld", """
+        r = Em2dRun(lib, dict(weibel(8, 8, 1), ext_fld={"B_0": [0, 0, 0.1]}))
+        lib.zpic_batch_new.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+        lib.zpic_batch_new((C.c_void_p * 1)(C.addressof(r.sim)), 1)""", "external fields"),
+    ("batch_twice", """
+        r = Em2dRun(lib, weibel(8, 8, 1))
+        lib.zpic_batch_new.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+        lib.zpic_batch_new((C.c_void_p * 2)(C.addressof(r.sim), C.addressof(r.sim)), 2)""", "appears twice"),
 ])
 def test_fatal_errors_exit_like_the_reference(lib, what, code, message):
     """Fatal input errors print a message and exit(-1) (simulation.c:106-110,

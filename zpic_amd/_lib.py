@@ -43,6 +43,9 @@ def load() -> C.CDLL:
         "zpic_kernel_timing": (None, [C.c_int]),
         "zpic_kernel_time": (C.c_double, [C.c_int, P(C.c_longlong)]),
         "zpic_kernel_timing_reset": (None, []),
+        "zpic_batch_new": (C.c_void_p, [P(C.c_void_p), C.c_int]),
+        "zpic_batch_run": (None, [C.c_void_p, C.c_int]),
+        "zpic_batch_delete": (None, [C.c_void_p]),
     }
     for name, (res, args) in ext.items():
         fn = getattr(lib, name)

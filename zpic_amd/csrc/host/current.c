@@ -99,12 +99,37 @@ static void compensator(int n, float *sa, float *sb)
     *sb = b / total;
 }
 
-static void smooth_pass(t_current *c, int axis, float sa, float sb)
+/* The filter passes of current_update in order: binomial passes, then the
+ * compensator, along x and then y; the y pass count is xlevel, as in the
+ * reference (em2d/current.c:427-459, 449).  Returns the number of passes. */
+int zpic_current_passes(const t_current *c, int axis[], float sa[], float sb[], int max)
 {
-    struct zpic_dev_current *d = c->dev;
-    ZCHECK(zk_smooth(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, axis, sa, sb,
-                     !c->moving_window, zpic_stream()));
-    d->jcur = 1 - d->jcur;
+    int n = 0;
+#define PASS(AX, A, B)                                                                                                 \
+    do {                                                                                                               \
+        if (n >= max) zpic_fatal("more than %d current filter passes", max);                                          \
+        axis[n] = (AX);                                                                                                \
+        sa[n] = (A);                                                                                                   \
+        sb[n] = (B);                                                                                                   \
+        n++;                                                                                                           \
+    } while (0)
+    float a, b;
+    if (c->smooth.xtype != NONE) {
+        for (int k = 0; k < c->smooth.xlevel; k++) PASS(0, 0.25f, 0.5f);
+        if (c->smooth.xtype == COMPENSATED) {
+            compensator(c->smooth.xlevel, &a, &b);
+            PASS(0, a, b);
+        }
+    }
+    if (c->smooth.ytype != NONE) {
+        for (int k = 0; k < c->smooth.xlevel; k++) PASS(1, 0.25f, 0.5f);
+        if (c->smooth.ytype == COMPENSATED) {
+            compensator(c->smooth.ylevel, &a, &b);
+            PASS(1, a, b);
+        }
+    }
+#undef PASS
+    return n;
 }
 
 void current_update(t_current *c)
@@ -120,23 +145,14 @@ void current_update(t_current *c)
     a.periodic_x = !c->moving_window;
     zpic_ktimer_begin(ZPIC_T_CURRENT);
     ZCHECK(zk_current_reduce(&a, zpic_stream()));
-
-    /* binomial passes, then the compensator; the y pass count is xlevel,
-     * as in the reference (em2d/current.c:449) */
-    float sa, sb;
-    if (c->smooth.xtype != NONE) {
-        for (int k = 0; k < c->smooth.xlevel; k++) smooth_pass(c, 0, 0.25f, 0.5f);
-        if (c->smooth.xtype == COMPENSATED) {
-            compensator(c->smooth.xlevel, &sa, &sb);
-            smooth_pass(c, 0, sa, sb);
-        }
-    }
-    if (c->smooth.ytype != NONE) {
-        for (int k = 0; k < c->smooth.xlevel; k++) smooth_pass(c, 1, 0.25f, 0.5f);
-        if (c->smooth.ytype == COMPENSATED) {
-            compensator(c->smooth.ylevel, &sa, &sb);
-            smooth_pass(c, 1, sa, sb);
-        }
+    const int maxp = 2 * (c->smooth.xlevel > 0 ? c->smooth.xlevel : 0) + 2;
+    int axis[maxp];
+    float sa[maxp], sb[maxp];
+    const int np = zpic_current_passes(c, axis, sa, sb, maxp);
+    for (int k = 0; k < np; k++) {
+        ZCHECK(zk_smooth(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, axis[k], sa[k], sb[k],
+                         !c->moving_window, zpic_stream()));
+        d->jcur = 1 - d->jcur;
     }
     zpic_ktimer_end(ZPIC_T_CURRENT);
     c->iter++;

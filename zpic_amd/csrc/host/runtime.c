@@ -244,3 +244,29 @@ void zpic_kernel_timing_reset(void)
         g_timer[w].count = 0;
     }
 }
+
+/* ---- HIP graphs of the batched step ---- */
+
+void zpic_capture_begin(void) { ZCHECK(hipStreamBeginCapture(zpic_stream(), hipStreamCaptureModeThreadLocal)); }
+
+void *zpic_capture_end(void)
+{
+    hipGraph_t g = NULL;
+    hipGraphExec_t x = NULL;
+    ZCHECK(hipStreamEndCapture(zpic_stream(), &g));
+    ZCHECK(hipGraphInstantiate(&x, g, NULL, NULL, 0));
+    ZCHECK(hipGraphDestroy(g));
+    return (void *)x;
+}
+
+void zpic_graph_launch(void *exec) { ZCHECK(hipGraphLaunch((hipGraphExec_t)exec, zpic_stream())); }
+
+void zpic_graph_free(void *exec)
+{
+    if (exec) ZCHECK(hipGraphExecDestroy((hipGraphExec_t)exec));
+}
+
+void zpic_h2d_async(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, zpic_stream()));
+}

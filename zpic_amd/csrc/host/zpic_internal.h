@@ -80,5 +80,6 @@ const float *zpic_emf_particle_B(t_emf *emf);
 const float *zpic_current_J(t_current *cur);
 
 void zpic_spec_sync_energy(t_species *s);
+int zpic_current_passes(const t_current *c, int axis[], float sa[], float sb[], int max);
 
 #endif

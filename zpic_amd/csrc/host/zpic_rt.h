@@ -25,6 +25,14 @@ void zpic_h2d(void *dst, const void *src, size_t bytes);      /* synchronous */
 void zpic_d2h(void *dst, const void *src, size_t bytes);      /* synchronous */
 void zpic_d2d_async(void *dst, const void *src, size_t bytes); /* on the stream */
 void zpic_sync(void);
+void zpic_h2d_async(void *dst, const void *src, size_t bytes); /* host buffer must outlive the copy */
+
+/* stream capture into a HIP graph; zpic_capture_end returns the
+ * instantiated graph, replayed on the stream by zpic_graph_launch */
+void zpic_capture_begin(void);
+void *zpic_capture_end(void);
+void zpic_graph_launch(void *exec);
+void zpic_graph_free(void *exec);
 void zpic_tiles_for(int nx, int ny, zk_tiles *t);
 int zpic_push_waves(void);
 size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */

@@ -14,104 +14,18 @@
  * zk_smooth applies one [sa, sb, sa] pass along x or y with the guard
  * handling of kernel_x / kernel_y (em2d/current.c:316-413).
  */
-#include "common.h"
+#include "current_core.h"
+
+using namespace zk;
 
 namespace {
 
-__device__ inline int floor_div(int a, int b)
-{
-    int q = a / b;
-    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
-}
-
-__global__ void reduce_kernel(zk_reduce_args a)
-{
-    const zk_tiles &t = a.t;
-    const int bw = t.nx + 3, bh = t.ny + 3;
-    const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= bw * bh) return;
-    const int i = node % bw - 1, j = node / bw - 1;
-    const int ci = a.periodic_x ? zk_wrap(i, t.nx) : i;
-    const int cj = zk_wrap(j, t.ny);
-    const int n = t.WX * t.WY;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int ky = -1; ky <= 1; ky++) {
-        const int ry = cj + ky * t.ny;
-        if (ry < -1 || ry > t.ny + 1) continue;
-        const int ty_lo = max(0, floor_div(ry + 1 + t.H - t.WY, t.TY) + 1);
-        const int ty_hi = min(t.nty - 1, floor_div(ry + 1 + t.H, t.TY));
-        for (int kx = -1; kx <= 1; kx++) {
-            if (!a.periodic_x && kx != 0) continue;
-            const int rx = ci + kx * t.nx;
-            if (rx < -1 || rx > t.nx + 1) continue;
-            const int tx_lo = max(0, floor_div(rx + 1 + t.H - t.WX, t.TX) + 1);
-            const int tx_hi = min(t.ntx - 1, floor_div(rx + 1 + t.H, t.TX));
-            for (int ty = ty_lo; a.region && ty <= ty_hi; ty++) {
-                const int v = ry - (ty * t.TY - 1 - t.H);
-                for (int tx = tx_lo; tx <= tx_hi; tx++) {
-                    const int u = rx - (tx * t.TX - 1 - t.H);
-                    const float *w = a.region + (size_t)(tx + ty * t.ntx) * 3 * n + u + v * t.WX;
-                    sx += w[0];
-                    sy += w[n];
-                    sz += w[2 * n];
-                }
-            }
-            const float *o = a.Jov + 3 * ((rx + 1) + (ry + 1) * a.nrow);
-            sx += o[0];
-            sy += o[1];
-            sz += o[2];
-        }
-    }
-    float *out = a.J + 3 * (size_t)node;
-    out[0] = sx;
-    out[1] = sy;
-    out[2] = sz;
-}
-
-__device__ inline void stencil(float sa, float sb, const float *l, const float *m, const float *u, float *o)
-{
-    o[0] = sa * l[0] + sb * m[0] + sa * u[0];
-    o[1] = sa * l[1] + sb * m[1] + sa * u[1];
-    o[2] = sa * l[2] + sb * m[2] + sa * u[2];
-}
+__global__ void reduce_kernel(zk_reduce_args a) { reduce_node(a, blockIdx.x * blockDim.x + threadIdx.x); }
 
 __global__ void smooth_kernel(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nrow,
                               int axis, float sa, float sb, int periodic_x)
 {
-    const int bw = nx + 3, bh = ny + 3;
-    const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= bw * bh) return;
-    const int i = node % bw - 1, j = node / bw - 1;
-#define AT(a, b) (in + 3 * ((size_t)((a) + 1) + (size_t)((b) + 1) * nrow))
-    float *o = out + 3 * (size_t)node;
-    const bool ix_in = (i >= 0 && i < nx), jy_in = (j >= 0 && j < ny);
-    if (axis == 0) {
-        /* kernel_x: interior rows only; x guards of those rows are copies
-         * unless the window moves */
-        if (jy_in && (ix_in || periodic_x)) {
-            const int c = ix_in ? i : zk_wrap(i, nx);
-            stencil(sa, sb, AT(c - 1, j), AT(c, j), AT(c + 1, j), o);
-            return;
-        }
-    } else {
-        /* kernel_y: interior nodes; then y guards of every column copy the
-         * final value of their image row */
-        const int c = jy_in ? j : zk_wrap(j, ny);
-        if (ix_in) {
-            stencil(sa, sb, AT(i, c - 1), AT(i, c), AT(i, c + 1), o);
-            return;
-        }
-        const float *s = AT(i, c);
-        o[0] = s[0];
-        o[1] = s[1];
-        o[2] = s[2];
-        return;
-    }
-    const float *s = AT(i, j);
-    o[0] = s[0];
-    o[1] = s[1];
-    o[2] = s[2];
-#undef AT
+    smooth_node(in, out, nx, ny, nrow, axis, sa, sb, periodic_x, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 } // namespace

@@ -1,0 +1,423 @@
+/*
+ * push_core.h - device code of the particle push shared by the
+ * one-simulation kernels (push.hip) and the batched kernels (batch.hip).
+ * See push.hip for the description of the algorithm.
+ */
+#ifndef ZK_PUSH_CORE_H
+#define ZK_PUSH_CORE_H
+
+#include "common.h"
+
+
+namespace zk {
+
+struct Seg {
+    float x0, x1, y0, y1, dx, dy, qvz;
+    int ix, iy;
+};
+
+/* Field sources: node (a, b) in the source's index space. */
+struct LdsFields {
+    const float *f;
+    int w, n;
+    __device__ float ex(int a, int b) const { return f[0 * n + a + b * w]; }
+    __device__ float ey(int a, int b) const { return f[1 * n + a + b * w]; }
+    __device__ float ez(int a, int b) const { return f[2 * n + a + b * w]; }
+    __device__ float bx(int a, int b) const { return f[3 * n + a + b * w]; }
+    __device__ float by(int a, int b) const { return f[4 * n + a + b * w]; }
+    __device__ float bz(int a, int b) const { return f[5 * n + a + b * w]; }
+};
+
+struct GlobalFields {
+    const float *E, *B;
+    int nrow;
+    __device__ float ex(int a, int b) const { return E[3 * (a + b * nrow) + 0]; }
+    __device__ float ey(int a, int b) const { return E[3 * (a + b * nrow) + 1]; }
+    __device__ float ez(int a, int b) const { return E[3 * (a + b * nrow) + 2]; }
+    __device__ float bx(int a, int b) const { return B[3 * (a + b * nrow) + 0]; }
+    __device__ float by(int a, int b) const { return B[3 * (a + b * nrow) + 1]; }
+    __device__ float bz(int a, int b) const { return B[3 * (a + b * nrow) + 2]; }
+};
+
+/* Current sinks. */
+struct LdsCurrent {
+    float *j;
+    int w, n;
+    __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], v); }
+};
+
+struct GlobalCurrent {
+    float *J;
+    int nrow;
+    __device__ void add(int a, int b, int c, float v) const
+    {
+        unsafeAtomicAdd(&J[3 * (a + b * nrow) + c], v);
+    }
+};
+
+/* Bilinear gather on the staggered mesh (em2d/particles.c:1029-1071);
+ * (i, j) is the particle's cell in the source index space. */
+template <class F>
+__device__ inline void gather(const F &S, int i, int j, float wx, float wy, float e[3], float b[3])
+{
+    const int ih = i + ((wx < 0.5f) ? -1 : 0);
+    const int jh = j + ((wy < 0.5f) ? -1 : 0);
+    const float wxh = wx + ((wx < 0.5f) ? 0.5f : -0.5f);
+    const float wyh = wy + ((wy < 0.5f) ? 0.5f : -0.5f);
+    e[0] = (S.ex(ih, j) * (1.0f - wxh) + S.ex(ih + 1, j) * wxh) * (1.0f - wy) +
+           (S.ex(ih, j + 1) * (1.0f - wxh) + S.ex(ih + 1, j + 1) * wxh) * wy;
+    e[1] = (S.ey(i, jh) * (1.0f - wx) + S.ey(i + 1, jh) * wx) * (1.0f - wyh) +
+           (S.ey(i, jh + 1) * (1.0f - wx) + S.ey(i + 1, jh + 1) * wx) * wyh;
+    e[2] = (S.ez(i, j) * (1.0f - wx) + S.ez(i + 1, j) * wx) * (1.0f - wy) +
+           (S.ez(i, j + 1) * (1.0f - wx) + S.ez(i + 1, j + 1) * wx) * wy;
+    b[0] = (S.bx(i, jh) * (1.0f - wx) + S.bx(i + 1, jh) * wx) * (1.0f - wyh) +
+           (S.bx(i, jh + 1) * (1.0f - wx) + S.bx(i + 1, jh + 1) * wx) * wyh;
+    b[1] = (S.by(ih, j) * (1.0f - wxh) + S.by(ih + 1, j) * wxh) * (1.0f - wy) +
+           (S.by(ih, j + 1) * (1.0f - wxh) + S.by(ih + 1, j + 1) * wxh) * wy;
+    b[2] = (S.bz(ih, jh) * (1.0f - wxh) + S.bz(ih + 1, jh) * wxh) * (1.0f - wyh) +
+           (S.bz(ih, jh + 1) * (1.0f - wxh) + S.bz(ih + 1, jh + 1) * wxh) * wyh;
+}
+
+/* Cut segment s at the y cell edge crossed (dj = +-1) into s and t
+ * (em2d/particles.c:838-879). */
+__device__ inline void ysplit(Seg &s, Seg &t, int dj)
+{
+    const int edge = (dj == 1);
+    const float f = (s.y1 - edge) / s.dy;
+    t.y0 = 1 - edge;
+    t.y1 = s.y1 - dj;
+    t.dy = s.dy * f;
+    t.iy = s.iy + dj;
+    const float xc = s.x0 + s.dx * (1.0f - f);
+    t.x0 = xc;
+    t.x1 = s.x1;
+    t.dx = s.dx * f;
+    t.ix = s.ix;
+    t.qvz = s.qvz * f;
+    s.y1 = edge;
+    s.dy *= (1.0f - f);
+    s.dx *= (1.0f - f);
+    s.x1 = xc;
+    s.qvz *= (1.0f - f);
+}
+
+/* Current of one straight segment inside one cell (em2d/particles.c:884-921). */
+template <class D>
+__device__ inline void deposit_seg(const D &J, const Seg &g, float qnx, float qny)
+{
+    const float ax0 = 1.0f - g.x0, bx0 = g.x0;
+    const float ax1 = 1.0f - g.x1, bx1 = g.x1;
+    const float ay0 = 1.0f - g.y0, by0 = g.y0;
+    const float ay1 = 1.0f - g.y1, by1 = g.y1;
+    const float jx = qnx * g.dx;
+    const float jy = qny * g.dy;
+    const float wy0 = 0.5f * (ay0 + ay1), wy1 = 0.5f * (by0 + by1);
+    const float wx0 = 0.5f * (ax0 + ax1), wx1 = 0.5f * (bx0 + bx1);
+    J.add(g.ix, g.iy, 0, jx * wy0);
+    J.add(g.ix, g.iy + 1, 0, jx * wy1);
+    J.add(g.ix, g.iy, 1, jy * wx0);
+    J.add(g.ix + 1, g.iy, 1, jy * wx1);
+    J.add(g.ix, g.iy, 2, g.qvz * (ax0 * ay0 + ax1 * ay1 + (ax0 * ay1 - ax1 * ay0) / 2.0f));
+    J.add(g.ix + 1, g.iy, 2, g.qvz * (bx0 * ay0 + bx1 * ay1 + (bx0 * ay1 - bx1 * ay0) / 2.0f));
+    J.add(g.ix, g.iy + 1, 2, g.qvz * (ax0 * by0 + ax1 * by1 + (ax0 * by1 - ax1 * by0) / 2.0f));
+    J.add(g.ix + 1, g.iy + 1, 2, g.qvz * (bx0 * by0 + bx1 * by1 + (bx0 * by1 - bx1 * by0) / 2.0f));
+}
+
+/* Split the motion at cell edges into up to three segments and deposit
+ * them in the reference's order (em2d/particles.c:773-924). */
+template <class D>
+__device__ inline void deposit(const D &J, int ix, int iy, int di, int dj, float x0, float y0,
+                               float dx, float dy, float qnx, float qny, float qvz)
+{
+    Seg s0, s1, s2;
+    s0.x0 = x0;
+    s0.y0 = y0;
+    s0.dx = dx;
+    s0.dy = dy;
+    s0.x1 = x0 + dx;
+    s0.y1 = y0 + dy;
+    s0.qvz = qvz * 0.5f; /* equals (float)(qvz / 2.0): halving is exact */
+    s0.ix = ix;
+    s0.iy = iy;
+    int n = 1;
+    if (di != 0) {
+        const int edge = (di == 1);
+        const float f = (x0 + dx - edge) / dx;
+        s1.x0 = 1 - edge;
+        s1.x1 = (x0 + dx) - di;
+        s1.dx = dx * f;
+        s1.ix = ix + di;
+        const float yc = y0 + dy * (1.0f - f);
+        s1.y0 = yc;
+        s1.y1 = s0.y1;
+        s1.dy = dy * f;
+        s1.iy = iy;
+        s1.qvz = s0.qvz * f;
+        s0.x1 = edge;
+        s0.dx *= (1.0f - f);
+        s0.dy *= (1.0f - f);
+        s0.y1 = yc;
+        s0.qvz *= (1.0f - f);
+        n = 2;
+    }
+    if (dj != 0) {
+        if (n == 1) {
+            ysplit(s0, s1, dj);
+        } else if (s0.y1 < 0.0f || s0.y1 >= 1.0f) {
+            /* the first segment crosses in y; the x-cut one lies beyond */
+            ysplit(s0, s2, dj);
+            s1.y0 -= dj;
+            s1.y1 -= dj;
+            s1.iy += dj;
+        } else {
+            ysplit(s1, s2, dj);
+        }
+        n++;
+    }
+    deposit_seg(J, s0, qnx, qny);
+    if (n > 1) deposit_seg(J, s1, qnx, qny);
+    if (n > 2) deposit_seg(J, s2, qnx, qny);
+}
+
+struct Consts {
+    float tem, dt_dx, dt_dy, qnx, qny, q;
+};
+
+struct Part {
+    int ix, iy;
+    float x, y, ux, uy, uz;
+};
+
+/* Particle streams are read and written once per step: nontemporal
+ * accesses keep them from evicting the field windows from the caches. */
+__device__ inline Part load_part(const zk_parts &P, int i)
+{
+    Part c;
+    c.ix = __builtin_nontemporal_load(&P.ix[i]);
+    c.iy = __builtin_nontemporal_load(&P.iy[i]);
+    c.x = __builtin_nontemporal_load(&P.x[i]);
+    c.y = __builtin_nontemporal_load(&P.y[i]);
+    c.ux = __builtin_nontemporal_load(&P.ux[i]);
+    c.uy = __builtin_nontemporal_load(&P.uy[i]);
+    c.uz = __builtin_nontemporal_load(&P.uz[i]);
+    return c;
+}
+
+__device__ inline void store_part(const zk_parts &P, int i, const Part &c)
+{
+    __builtin_nontemporal_store(c.ix, &P.ix[i]);
+    __builtin_nontemporal_store(c.iy, &P.iy[i]);
+    __builtin_nontemporal_store(c.x, &P.x[i]);
+    __builtin_nontemporal_store(c.y, &P.y[i]);
+    __builtin_nontemporal_store(c.ux, &P.ux[i]);
+    __builtin_nontemporal_store(c.uy, &P.uy[i]);
+    __builtin_nontemporal_store(c.uz, &P.uz[i]);
+}
+
+/* Push one particle whose cell is (ci, cj) in the sources' index space;
+ * returns its time-centred kinetic energy term (em2d/particles.c:1125-1228). */
+template <class F, class D>
+__device__ inline double push_one(const F &S, const D &J, int ci, int cj, float &x, float &y,
+                                  float &ux, float &uy, float &uz, int &di, int &dj, const Consts &k)
+{
+    float e[3], b[3];
+    gather(S, ci, cj, x, y, e, b);
+    e[0] *= k.tem;
+    e[1] *= k.tem;
+    e[2] *= k.tem;
+    float ax = ux + e[0], ay = uy + e[1], az = uz + e[2];
+    const float usq = ax * ax + ay * ay + az * az;
+    const float g = sqrtf(1.0f + usq);
+    const double kin = (double)(usq / (g + 1.0f));
+    const float tg = k.tem / g;
+    b[0] *= tg;
+    b[1] *= tg;
+    b[2] *= tg;
+    const float s2 = 2.0f / (1.0f + b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    const float vx = ax + ay * b[2] - az * b[1];
+    const float vy = ay + az * b[0] - ax * b[2];
+    const float vz = az + ax * b[1] - ay * b[0];
+    b[0] *= s2;
+    b[1] *= s2;
+    b[2] *= s2;
+    ax += vy * b[2] - vz * b[1];
+    ay += vz * b[0] - vx * b[2];
+    az += vx * b[1] - vy * b[0];
+    ux = ax + e[0];
+    uy = ay + e[1];
+    uz = az + e[2];
+
+    const float rg = 1.0f / sqrtf(1.0f + ux * ux + uy * uy + uz * uz);
+    const float mx = k.dt_dx * rg * ux;
+    const float my = k.dt_dy * rg * uy;
+    float x1 = x + mx, y1 = y + my;
+    di = (x1 >= 1.0f) - (x1 < 0.0f);
+    dj = (y1 >= 1.0f) - (y1 < 0.0f);
+    x1 -= di;
+    y1 -= dj;
+    const float qvz = k.q * uz * rg;
+    deposit(J, ci, cj, di, dj, x, y, mx, my, k.qnx, k.qny, qvz);
+    x = x1;
+    y = y1;
+    return kin;
+}
+
+/* Boundary conditions after the push (em2d/particles.c:1236-1259); the
+ * window shift of spec_move_window (:619-640) is applied first. */
+__device__ inline bool boundaries(int &ix, int &iy, int nx, int ny, int mw, int shift)
+{
+    if (mw) {
+        ix -= shift;
+        if (ix < 0 || ix >= nx) {
+            ix = ZK_DEAD;
+            return true;
+        }
+    } else {
+        ix += ((ix < 0) ? nx : 0) - ((ix >= nx) ? nx : 0);
+    }
+    iy += ((iy < 0) ? ny : 0) - ((iy >= ny) ? ny : 0);
+    return false;
+}
+
+__device__ inline void push_tile_body(const zk_push_args &a, const int tile)
+{
+    extern __shared__ float sm[];
+    __shared__ double red[16];
+    __shared__ int ndead_blk;
+    const zk_tiles &t = a.t;
+    const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
+    zk_parts P = a.p;
+    double kin = 0.0;
+    int ndead = 0;
+    if (threadIdx.x == 0) ndead_blk = 0;
+
+    const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
+    const int n = t.WX * t.WY;
+    const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H; /* cell index of window node 0 */
+    float *fl = sm;
+    float *jl = sm + 6 * n;
+    /* stage the window of E and B; nodes outside the guard-inclusive
+     * buffer are never read by an in-window particle */
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        const int gx = ox + m % t.WX, gy = oy + m / t.WX;
+        const bool in = gx >= -1 && gx <= t.nx + 1 && gy >= -1 && gy <= t.ny + 1;
+        const int g = 3 * ((gx + 1) + (gy + 1) * a.nrow);
+        fl[0 * n + m] = in ? a.E[g + 0] : 0.0f;
+        fl[1 * n + m] = in ? a.E[g + 1] : 0.0f;
+        fl[2 * n + m] = in ? a.E[g + 2] : 0.0f;
+        fl[3 * n + m] = in ? a.B[g + 0] : 0.0f;
+        fl[4 * n + m] = in ? a.B[g + 1] : 0.0f;
+        fl[5 * n + m] = in ? a.B[g + 2] : 0.0f;
+        jl[0 * n + m] = 0.0f;
+        jl[1 * n + m] = 0.0f;
+        jl[2 * n + m] = 0.0f;
+    }
+    __syncthreads();
+    const LdsFields lf = {fl, t.WX, n};
+    const LdsCurrent lj = {jl, t.WX, n};
+    const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
+    const int lane = __lane_id();
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        Part c = load_part(P, i);
+        const int lx = c.ix - tx0, ly = c.iy - ty0;
+        const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
+        const bool live = c.ix != ZK_DEAD;
+        /* list the live particles outside the window, one atomic per wave */
+        const unsigned long long out = __ballot(live && !inside);
+        if (out) {
+            int base = 0;
+            const int leader = __ffsll((long long)out) - 1;
+            if (lane == leader) base = atomicAdd(&a.ctr->n_listed, __popcll(out));
+            base = __shfl(base, leader, ZK_WAVE);
+            if (live && !inside) a.ovf_list[base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+        }
+        if (!live || !inside) continue;
+        int di, dj;
+        kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+        c.ix += di;
+        c.iy += dj;
+        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+        store_part(P, i, c);
+    }
+    __syncthreads();
+    /* store the current window of this tile */
+    float *dst = a.region + (size_t)tile * 3 * n;
+    if (a.fresh) {
+        for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] = jl[m];
+    } else {
+        for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] += jl[m];
+    }
+    /* per-tile energy partial (summed in fixed order by zk_spec_finalize) */
+    const double s = zk_block_sum(kin, red);
+    if (threadIdx.x == 0) a.epart[tile] = s;
+    if (ndead) atomicAdd(&ndead_blk, ndead);
+    __syncthreads();
+    if (threadIdx.x == 0 && ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
+}
+
+/* Global pass: the particles listed by the tile pass, then the unsorted
+ * tail (injected since the last sort), with gathers from the guard-inclusive
+ * field grids as the reference indexes them and atomics into the raw
+ * overflow current. */
+__device__ inline void push_global_body(const zk_push_args &a, const int block, const int nblocks)
+{
+    __shared__ double red[16];
+    __shared__ int ndead_blk, nov_blk;
+    const zk_tiles &t = a.t;
+    const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
+    const GlobalFields gf = {a.E, a.B, a.nrow};
+    const GlobalCurrent gj = {a.Jov, a.nrow};
+    zk_parts P = a.p;
+    double kin = 0.0;
+    int ndead = 0, nov = 0;
+    if (threadIdx.x == 0) ndead_blk = nov_blk = 0;
+    const int listed = a.ctr->n_listed;
+    const int tail0 = a.ctr->n_sorted;
+    const int total = listed + (a.ctr->np_total - tail0);
+    for (int q = block * blockDim.x + threadIdx.x; q < total; q += nblocks * blockDim.x) {
+        const int i = q < listed ? a.ovf_list[q] : tail0 + (q - listed);
+        Part c = load_part(P, i);
+        if (c.ix == ZK_DEAD) continue;
+        int di, dj;
+        nov++;
+        kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+        c.ix += di;
+        c.iy += dj;
+        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+        store_part(P, i, c);
+    }
+    const double s = zk_block_sum(kin, red);
+    if (threadIdx.x == 0) a.epart[t.ntiles + block] = s;
+    if (ndead) atomicAdd(&ndead_blk, ndead);
+    if (nov) atomicAdd(&nov_blk, nov);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
+        if (nov_blk) atomicAdd(&a.ctr->n_overflow, nov_blk);
+    }
+}
+
+/* Sum the energy partials in block order and count the pushes
+ * (em2d/particles.c:1231, 1267). */
+__device__ inline void finalize_body(zk_spec_ctr *ctr, const double *epart, int nparts, double qm, double dx0,
+                                     double dx1)
+{
+    __shared__ double part[1024];
+    double s = 0.0;
+    const int per = (nparts + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per, e = min(nparts, b + per);
+    for (int i = b; i < e; i++) s += epart[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int i = 0; i < (int)blockDim.x; i++) tot += part[i];
+        ctr->energy = qm * tot * dx0 * dx1;
+        ctr->npush += (unsigned long long)(ctr->np_total - ctr->n_dead);
+        ctr->n_listed = 0;
+    }
+}
+
+} // namespace zk
+
+#endif

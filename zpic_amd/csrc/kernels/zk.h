@@ -117,6 +117,57 @@ int zk_emf_energy(const float *E, const float *B, int nx, int ny, int nrow,
 int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, float q,
                      int nx, float *rho, zk_stream_t s);
 
+/* ---- batches of independent simulations (batch.hip) ---- */
+#define ZK_MAX_PASSES 16
+
+/* Yee constants: (dt/2)/dx, (dt/2)/dy, dt/dx, dt/dy, dt */
+typedef struct zk_yee_k {
+    float dtx2, dty2, dtx, dty, dt;
+} zk_yee_k;
+
+/* One simulation of a batch.  ecur/jcur select the current E,B and J
+ * buffers and are advanced on the device at the end of every step. */
+typedef struct zk_bsim {
+    float *E[2], *B[2], *J[2];
+    float *region, *Jov;
+    int ecur, jcur, npass, deposited;
+    int nx, ny, nrow;
+    zk_tiles t;
+    zk_yee_k yk;
+    int pass_axis[ZK_MAX_PASSES];
+    float pass_sa[ZK_MAX_PASSES], pass_sb[ZK_MAX_PASSES];
+} zk_bsim;
+
+/* One species of a batch: the static push arguments (a.global_blocks is
+ * its block count in the out-of-window pass), both particle buffer sets
+ * (pcur selects the current one), its simulation, its sort arrays and
+ * schedule: it is sorted after the push that makes iter + 1 a multiple
+ * of n_sort; iter counts the steps done, advanced on the device. */
+typedef struct zk_bspec {
+    zk_push_args a;
+    zk_parts set[2];
+    int pcur, sim, nparts, iter, n_sort, sort_blocks;
+    int *off, *cnt, *cursor; /* off is a.tile_off, writable */
+    double qm, dx0, dx1;
+} zk_bspec;
+
+/* Item tables hold (descriptor index, block index) pairs of ints; the
+ * launches are one workgroup per item. */
+int zk_batch_fill_jov(const zk_bsim *sims, const int *items, int n, zk_stream_t s);
+int zk_batch_prepare_push(size_t lds);
+int zk_batch_push(const zk_bspec *specs, const zk_bsim *sims, const int *tile_items, int ntile,
+                  const int *global_items, int nglobal, size_t lds, int waves, zk_stream_t s);
+int zk_batch_finalize(const zk_bspec *specs, int nspecs, zk_stream_t s);
+int zk_batch_current(const zk_bsim *sims, const int *node_items, int n, int max_passes, zk_stream_t s);
+int zk_batch_fields(const zk_bsim *sims, const int *yee_items, int nyee, const int *guard_items, int nguard,
+                    zk_stream_t s);
+int zk_batch_sort(const zk_bspec *specs, int nspecs, const int *items, int n, zk_stream_t s);
+int zk_batch_end_step(zk_bsim *sims, int nsims, zk_bspec *specs, int nspecs, zk_stream_t s);
+int zk_batch_end_step_sorted(zk_bsim *sims, int nsims, zk_bspec *specs, int nspecs, zk_stream_t s);
+/* output-node tiles of the Yee kernel */
+#define ZK_YEE_BX 32
+#define ZK_YEE_BY 16
+
 #ifdef __cplusplus
 }
 #endif

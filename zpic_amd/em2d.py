@@ -572,3 +572,48 @@ class Simulation:
     @property
     def box(self):
         return self._emf.box
+
+
+class Batch:
+    """Independent simulations advanced together on the GPU (parameter
+    scans).  No counterpart in the reference module; the reference runs a
+    scan's simulations one after another (mods/em1d_param_scan/main.c:50-98).
+
+    `run(steps)` leaves every simulation exactly where `steps` calls of its
+    own `iter()` would; each step of all simulations is one replay of a
+    captured HIP graph.  Simulations must be periodic, without moving
+    window or external fields; their grid, species and filter must not
+    change while the batch exists.
+    """
+
+    def __init__(self, sims):
+        self._sims = list(sims)
+        if not self._sims:
+            raise ValueError("Batch needs at least one simulation")
+        lib = _lib_()
+        arr = (C.c_void_p * len(self._sims))(*[C.addressof(s._sim) for s in self._sims])
+        self._b = lib.zpic_batch_new(arr, len(self._sims))
+
+    def run(self, steps: int):
+        """Advance every simulation by `steps` iterations."""
+        if self._b is None:
+            raise RuntimeError("batch is closed")
+        _lib_().zpic_batch_run(self._b, int(steps))
+        for s in self._sims:
+            s._n += int(steps)
+            s._t = s._n * s._sim.dt
+
+    def close(self):
+        if self._b is not None:
+            _lib_().zpic_batch_delete(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def simulations(self):
+        return self._sims
