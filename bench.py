@@ -88,6 +88,97 @@ def copy_probe(nbytes: int) -> int:
     return 0
 
 
+def member_energy(m) -> float:
+    """Total field + kinetic energy of a scan member (sim_report_energy's sum)."""
+    import ctypes as C
+
+    lib = m.lib
+    lib.sim_sync_host(C.byref(m.sim))
+    e6 = (C.c_double * 6)()
+    lib.emf_get_energy(C.byref(m.sim.emf), e6)
+    return float(sum(e6[:]) + m.species[0].energy + m.species[1].energy)
+
+
+def batch_workload(args) -> dict:
+    """Parameter scan of `--members` two-stream / Weibel simulations of
+    uneven size (zpic_amd.scan), `--batch-steps` steps each: one batched
+    run on the GPU against the same members run one after another with
+    sim_iter (each built, run and freed in turn)."""
+    import ctypes as C
+
+    from zpic_amd import _lib
+    from zpic_amd import scan
+
+    lib = _lib.load()
+    lib.zpic_set_push_waves(args.waves)
+    seeds = list(range(args.members))
+    steps = args.batch_steps
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    log(f"batch: building {len(seeds)} members on {threads} host threads")
+    t0 = time.time()
+    members = scan.build_members(seeds, threads=threads)
+    npart = sum(m.particles for m in members)
+    log(f"batch: init {time.time() - t0:.1f} s, {npart} particles; uploading")
+    t0 = time.time()
+    b = scan.Batch(members)
+    lib.zpic_synchronize()
+    log(f"batch: batch set up in {time.time() - t0:.1f} s; running {steps} steps")
+    t0 = time.perf_counter()
+    for done in range(0, steps, 50):
+        b.run(min(50, steps - done))
+        lib.zpic_synchronize()
+        log(f"batch: {min(done + 50, steps)} steps")
+    t_batch = time.perf_counter() - t0
+    e_batch = [member_energy(m) for m in members]
+    b.close()
+    for m in members:
+        m.close()
+    del members
+
+    log("batch: sequential runs")
+    t_seq = 0.0
+    e_seq = []
+    for s in seeds[::args.seq_stride]:
+        m = scan.Member(lib, scan.member_params(s))
+        m.step(1)  # upload and first step outside the timed region, as for the batch
+        lib.zpic_synchronize()
+        t0 = time.perf_counter()
+        m.step(steps - 1)
+        lib.zpic_synchronize()
+        t_seq += time.perf_counter() - t0
+        e_seq.append(member_energy(m))
+        m.close()
+        if len(e_seq) % 64 == 0:
+            log(f"batch: {len(e_seq)} sequential members")
+    nseq = len(e_seq)
+    # the sequential runs time steps 2..N; scale to N steps per member
+    seq_rate = nseq / (t_seq * steps / (steps - 1))
+    rel = [abs(a - c) / abs(c) for a, c in zip(e_batch[::args.seq_stride], e_seq) if c != 0]
+    return {
+        "metric": "batched sims/s",
+        "value": len(seeds) / t_batch,
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * t_batch / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded two-stream / Weibel members from the em2d injection",
+        "config": {"workload": (f"parameter scan: {len(seeds)} two-stream/Weibel members, grids "
+                                f"{{64,128,256}}^2 (independent x, y), 4-64 ppc per species, uz U[0.1,0.6], "
+                                f"uth U[0.01,0.1], seeds 0..{len(seeds) - 1}, {steps} steps each"),
+                   "members": len(seeds), "particles": npart},
+        "particle_steps_per_s": npart * steps / t_batch,
+        "sequential": {"value": seq_rate, "unit": "sims/s", "members_run": nseq,
+                       "note": "same members, one after another with sim_iter on the GPU"},
+        "speedup_vs_sequential": (len(seeds) / t_batch) / seq_rate,
+        "energy_rel_diff_max": max(rel) if rel else None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +194,12 @@ def main() -> int:
     ap.add_argument("--cpu-nx", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--workload", choices=["weibel", "batch"], default="weibel",
+                    help="weibel: the 2048^2 Weibel step (default); batch: the batched parameter scan")
+    ap.add_argument("--members", type=int, default=512, help="batch: members of the scan")
+    ap.add_argument("--batch-steps", type=int, default=300, help="batch: steps per member")
+    ap.add_argument("--seq-stride", type=int, default=1, help="batch: run every n-th member sequentially")
+    ap.add_argument("--threads", type=int, default=0, help="batch: host threads for init (0: min(16, cores))")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES through the HIP runtime "
                          "(calibrates the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -117,6 +214,9 @@ def main() -> int:
         return 2
 
     ensure_built()
+    if args.workload == "batch":
+        print(json.dumps(batch_workload(args)), flush=True)
+        return 0
     import ctypes as C
 
     from zpic_amd import _lib

@@ -165,7 +165,11 @@ void spec_grow_buffer(t_species *s, const int size)
 
 /* Thermal momenta, per-cell mean removed, plus the fluid drift
  * (em2d/particles.c:85-149).  The cell index uses nx[1] as its row stride,
- * exactly as the reference does. */
+ * exactly as the reference does.  The reference sizes its per-cell arrays
+ * nx[0] * nx[1] (:104-108), so for nx[1] > nx[0] its indices run past them
+ * (undefined behaviour); here the arrays cover every index the stride
+ * makes, nx[1] * max(nx[0], nx[1]), which gives the reference's values
+ * wherever its indices stay in bounds. */
 static void inject_momenta(t_species *s, int first, int last)
 {
     for (int i = first; i <= last; i++) {
@@ -173,8 +177,8 @@ static void inject_momenta(t_species *s, int first, int last)
         s->part[i].uy = s->uth[1] * rand_norm();
         s->part[i].uz = s->uth[2] * rand_norm();
     }
-    const int ncell = s->nx[0] * s->nx[1];
     const int stride = s->nx[1];
+    const int ncell = stride * (s->nx[0] > s->nx[1] ? s->nx[0] : s->nx[1]);
     float *mean = calloc((size_t)ncell * 3, sizeof(float));
     int *count = calloc((size_t)ncell, sizeof(int));
     for (int i = first; i <= last; i++) {

@@ -377,3 +377,30 @@ def test_oracle_ids_follow_particles(oracle):
         assert (np.minimum(d, 16 - d) <= 2).all()
     finally:
         run.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 32), (32, 16), (24, 24)])
+def test_initial_momenta_remove_the_mean_per_stride_key(oracle, nx, ny):
+    """spec_set_u removes the mean thermal momentum of the particles that
+    share the key ix + nx[1] * iy (em2d/particles.c:103-142): the plain
+    per-cell mean for square grids and for nx[1] > nx[0] (where the keys are
+    distinct; the reference's arrays are too small there), aliased cells
+    for nx[0] > nx[1].  Checked by grouping the initial particles by key."""
+    run = Em2dRun(oracle, weibel(nx, ny, 3, uz=0.6, uth=0.1))
+    try:
+        p = run.state()["part0"]
+        key = p["ix"].astype(np.int64) + ny * p["iy"].astype(np.int64)
+        for comp, fluid in (("ux", 0.0), ("uy", 0.0), ("uz", 0.6)):
+            th = p[comp].astype(np.float64) - fluid
+            sums = np.bincount(key, weights=th)
+            cnt = np.bincount(key)
+            used = cnt > 0
+            assert np.abs(sums[used] / cnt[used]).max() < 2e-6, comp
+        # the group sizes tell the aliasing apart from plain cells
+        sizes = np.unique(np.bincount(key)[np.bincount(key) > 0])
+        expect = {9} if nx <= ny else set(sizes.tolist())
+        assert set(sizes.tolist()) == expect
+        if nx > ny:
+            assert sizes.max() > 9  # some keys gather several cells
+    finally:
+        run.close()

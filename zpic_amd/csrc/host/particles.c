@@ -9,6 +9,7 @@
  * memory, grouped by tile.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -17,9 +18,12 @@
 #include "timer.h"
 #include "zpic_internal.h"
 
-/* all live species, for the process-wide push counter */
+/* all live species, for the process-wide push counter; spec_new may run
+ * on several host threads at once (independent simulations initialised
+ * in parallel), so the list is guarded */
 static struct zpic_dev_species *g_live = NULL;
 static unsigned long long g_npush_deleted = 0;
+static pthread_mutex_t g_live_lock = PTHREAD_MUTEX_INITIALIZER;
 
 /* ------------------------------------------------------------------------
  * injection on the host (em2d/particles.c:85-492)
@@ -160,7 +164,10 @@ static void inject_momenta(const t_species *s, t_part *p, int n)
         p[i].uy = s->uth[1] * rand_norm();
         p[i].uz = s->uth[2] * rand_norm();
     }
-    const int cells = s->nx[0] * s->nx[1];
+    /* the row stride is nx[1], as in the reference; the arrays cover every
+     * index it makes (the reference's nx[0] * nx[1] arrays are overrun
+     * when nx[1] > nx[0]) */
+    const int cells = s->nx[1] * (s->nx[0] > s->nx[1] ? s->nx[0] : s->nx[1]);
     float3 *mean = calloc((size_t)cells, sizeof(float3));
     int *count = calloc((size_t)cells, sizeof(int));
     for (int i = 0; i < n; i++) {
@@ -252,8 +259,10 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
     d->host_dirty = 1;
     d->host_valid = 1;
     d->owner = s;
+    pthread_mutex_lock(&g_live_lock);
     d->next_live = g_live;
     g_live = d;
+    pthread_mutex_unlock(&g_live_lock);
     s->dev = d;
 }
 
@@ -305,10 +314,12 @@ void spec_delete(t_species *s)
     if (d) {
         zpic_sync();
         species_free_device(d);
+        pthread_mutex_lock(&g_live_lock);
         g_npush_deleted += d->npush_retired;
         struct zpic_dev_species **pp = &g_live;
         while (*pp && *pp != d) pp = &(*pp)->next_live;
         if (*pp) *pp = d->next_live;
+        pthread_mutex_unlock(&g_live_lock);
         free(d->host_ids);
         free(d);
         s->dev = NULL;
@@ -320,8 +331,10 @@ void spec_delete(t_species *s)
 
 uint64_t spec_npush(void)
 {
+    pthread_mutex_lock(&g_live_lock);
     unsigned long long n = g_npush_deleted;
     for (struct zpic_dev_species *d = g_live; d; d = d->next_live) n += device_npush(d);
+    pthread_mutex_unlock(&g_live_lock);
     return n;
 }
 
