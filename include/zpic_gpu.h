@@ -106,6 +106,27 @@ zpic_batch *zpic_batch_new(t_simulation *const *sims, int n);
 void zpic_batch_run(zpic_batch *batch, int steps);
 void zpic_batch_delete(zpic_batch *batch);
 
+/* Differentiable runs.  No counterpart in the reference.
+ *
+ * zpic_diff_new prepares `steps` iterations of `sim` (periodic, no moving
+ * window, no external fields, no current filter) from its present state,
+ * with a checkpoint every `ckpt` steps; the particles are tile-sorted at
+ * each checkpoint instead of on the species' own schedule.  The loss is
+ *     L = sum over n in [loss_first, loss_last] of dt * (E_B3 + E_B4 + E_B5)(n)
+ * with E_B the six terms of emf_get_energy after n of the steps (FP64).
+ * zpic_diff_forward runs the steps and returns L; zpic_diff_backward then
+ * returns dL/d(initial state): dE and dB as guard-inclusive grids laid out
+ * like E_buf/B_buf ((nx+3)*(ny+3) float3), and for species k, dpart[k]
+ * holds 5 floats per particle (x, y, ux, uy, uz; position adjoints are
+ * per cell width) in the order of spec->part when zpic_diff_new was
+ * called.  Any pointer may be NULL.  Afterwards the simulation is in its
+ * state after the forward run. */
+typedef struct zpic_diff zpic_diff;
+zpic_diff *zpic_diff_new(t_simulation *sim, int steps, int ckpt, int loss_first, int loss_last);
+double zpic_diff_forward(zpic_diff *run);
+void zpic_diff_backward(zpic_diff *run, float *dE, float *dB, float *const *dpart);
+void zpic_diff_delete(zpic_diff *run);
+
 #ifdef __cplusplus
 }
 #endif

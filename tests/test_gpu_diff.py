@@ -1,0 +1,156 @@
+"""The GPU adjoint (zpic_diff_*, zpic_amd.diff) against an independent FP64
+gradient (torch autograd through oracle/diff64.py) and against central
+finite differences of the GPU forward run.
+
+Differentiable runs have no counterpart in the reference; the forward run
+is the GPU time step, pinned to the oracle by the parity tests, and the
+FP64 restatement is pinned to the oracle by tests/test_diff.py.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from conftest import ROOT
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import diff64  # noqa: E402
+
+from zpic_amd.diff import WeibelConfig, WeibelRun, weibel_loss  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def reference(run: WeibelRun, uz, uth, seed, leaves="params"):
+    """FP64 loss of the same run through the torch restatement; leaves are
+    either the parameters (uz, uth, seed) or the full initial state."""
+    cfg = run.cfg
+    grid = diff64.Grid(cfg.nx, cfg.ny, cfg.dx, cfg.dx, cfg.dt)
+    jj, ii = run.block_index()
+    species = []
+    if leaves == "params":
+        tuz = torch.tensor(uz, dtype=torch.float64, requires_grad=True)
+        tuth = torch.tensor(uth, dtype=torch.float64, requires_grad=True)
+        tseed = torch.tensor(np.asarray(seed, np.float64), requires_grad=True)
+        for k in range(2):
+            p = run.initial[k]
+            g = torch.tensor(run.G[k], dtype=torch.float64)
+            drift = torch.stack([torch.zeros((), dtype=torch.float64)] * 2 + [run.sign[k] * tuz])
+            species.append(diff64.Species(run.q[k], -1.0, torch.tensor(p["ix"]), torch.tensor(p["iy"]),
+                                          torch.tensor(p["x"], dtype=torch.float64),
+                                          torch.tensor(p["y"], dtype=torch.float64), tuth * g + drift))
+        bz = tseed[torch.tensor(np.broadcast_to(jj, (cfg.ny + 3, cfg.nx + 3)).copy()),
+                   torch.tensor(np.broadcast_to(ii, (cfg.ny + 3, cfg.nx + 3)).copy())]
+        B = torch.nn.functional.pad(bz[..., None], (2, 0))
+        E = grid.zeros()
+        L = diff64.run_loss(grid, species, E, B, cfg.steps, cfg.loss_first, cfg.loss_last)[0]
+        L.backward()
+        return float(L), (float(tuz.grad), float(tuth.grad), tseed.grad.numpy())
+    leaves_x = []
+    for k in range(2):
+        p = run.initial[k]
+        x = torch.tensor(p["x"], dtype=torch.float64, requires_grad=True)
+        y = torch.tensor(p["y"], dtype=torch.float64, requires_grad=True)
+        u = torch.tensor(np.stack([p["ux"], p["uy"], p["uz"]], 1), dtype=torch.float64, requires_grad=True)
+        leaves_x.append((x, y, u))
+        species.append(diff64.Species(run.q[k], -1.0, torch.tensor(p["ix"]), torch.tensor(p["iy"]), x, y, u))
+    B0 = np.zeros((cfg.ny + 3, cfg.nx + 3, 3))
+    B0[..., 2] = np.asarray(seed, np.float32)[jj, ii]
+    B = torch.tensor(B0, requires_grad=True)
+    E = torch.zeros_like(B, requires_grad=True)
+    L = diff64.run_loss(grid, species, E, B, cfg.steps, cfg.loss_first, cfg.loss_last)[0]
+    L.backward()
+    parts = [np.concatenate([x.grad.numpy()[:, None], y.grad.numpy()[:, None], u.grad.numpy()], 1)
+             for x, y, u in leaves_x]
+    return float(L), (E.grad.numpy(), B.grad.numpy(), parts)
+
+
+def test_short_run_adjoint_matches_fp64_state_gradient():
+    """Four steps with a checkpoint every two: the adjoint of every initial
+    field node and of every particle coordinate, against FP64 autograd."""
+    cfg = WeibelConfig(nx=32, ny=32, ppc=2, steps=4, ckpt=2, loss_first=1, loss_last=4)
+    seed = 1e-2 * np.random.default_rng(7).standard_normal((8, 8))
+    run = WeibelRun(cfg, 0.6, 0.1, seed)
+    try:
+        print("diff: short run forward", flush=True)
+        L = run.forward()
+        dE, dB, parts = run.backward()
+    finally:
+        run.close()
+    L64, (gE, gB, gparts) = reference(run, 0.6, 0.1, seed, leaves="state")
+    print(f"loss {L:.9e} fp64 {L64:.9e}; dE {rel(dE, gE):.2e} dB {rel(dB, gB):.2e} "
+          f"parts {[rel(parts[k], gparts[k]) for k in range(2)]}", flush=True)
+    assert abs(L - L64) <= 1e-5 * abs(L64)
+    assert rel(dB, gB) <= 1e-4
+    assert rel(dE, gE) <= 1e-4
+    for k in range(2):
+        assert rel(parts[k], gparts[k]) <= 1e-4
+
+
+def test_weibel_subcase_gradients_match_fp64_reference():
+    """The differentiable Weibel loss on a 32 x 32 grid (200 steps,
+    checkpoints every 20, loss over steps 150-200, 8 x 8 seed of B_z):
+    GPU adjoint gradients against FP64 autograd through the restatement."""
+    cfg = WeibelConfig(nx=32, ny=32, ppc=4, steps=200, ckpt=20, loss_first=150, loss_last=200)
+    seed = 1e-3 * np.random.default_rng(11).standard_normal((8, 8))
+    uz, uth = 0.6, 0.1
+    t_uz = torch.tensor(uz, dtype=torch.float64, requires_grad=True)
+    t_uth = torch.tensor(uth, dtype=torch.float64, requires_grad=True)
+    t_seed = torch.tensor(seed, dtype=torch.float64, requires_grad=True)
+    print("diff: 32x32 subcase through torch.autograd.Function", flush=True)
+    L = weibel_loss(t_uz, t_uth, t_seed, cfg)
+    L.backward()
+    run = WeibelRun(cfg, uz, uth, seed)  # same initial state, for the reference
+    run.close()
+    L64, (g_uz, g_uth, g_seed) = reference(run, uz, uth, seed)
+    print(f"loss {float(L):.9e} fp64 {L64:.9e}; d/duz {float(t_uz.grad):.6e} vs {g_uz:.6e}; "
+          f"d/duth {float(t_uth.grad):.6e} vs {g_uth:.6e}; seed rel {rel(t_seed.grad.numpy(), g_seed):.2e}",
+          flush=True)
+    assert abs(float(L) - L64) <= 1e-3 * abs(L64)
+    assert abs(float(t_uz.grad) - g_uz) <= 1e-3 * abs(g_uz)
+    assert abs(float(t_uth.grad) - g_uth) <= 1e-3 * abs(g_uth)
+    assert rel(t_seed.grad.numpy(), g_seed) <= 1e-3
+
+
+def test_adjoint_matches_central_differences_of_gpu_runs():
+    """Central differences (h = 1e-3) of the GPU forward loss against the
+    adjoint gradient, for uz, uth and one seed cell."""
+    cfg = WeibelConfig(nx=64, ny=64, ppc=4, steps=200, ckpt=20, loss_first=150, loss_last=200)
+    seed = 1e-3 * np.random.default_rng(5).standard_normal((16, 16))
+    uz, uth, h = 0.6, 0.1, 1e-3
+
+    def loss(a, b, s):
+        r = WeibelRun(cfg, a, b, s)
+        try:
+            return r.forward()
+        finally:
+            r.close()
+
+    run = WeibelRun(cfg, uz, uth, seed)
+    try:
+        print("diff: 64x64 adjoint", flush=True)
+        run.forward()
+        g_uz, g_uth, g_seed = run.gradients()
+    finally:
+        run.close()
+    print("diff: 64x64 central differences", flush=True)
+    fd_uz = (loss(uz + h, uth, seed) - loss(uz - h, uth, seed)) / (2 * h)
+    fd_uth = (loss(uz, uth + h, seed) - loss(uz, uth - h, seed)) / (2 * h)
+    e = np.zeros_like(seed)
+    e[3, 9] = h
+    fd_s = (loss(uz, uth, seed + e) - loss(uz, uth, seed - e)) / (2 * h)
+    print(f"uz {g_uz:.6e} fd {fd_uz:.6e}; uth {g_uth:.6e} fd {fd_uth:.6e}; seed {g_seed[3, 9]:.6e} fd {fd_s:.6e}",
+          flush=True)
+    assert abs(g_uz - fd_uz) <= 1e-3 * abs(fd_uz)
+    assert abs(g_uth - fd_uth) <= 1e-3 * abs(fd_uth)
+    assert abs(g_seed[3, 9] - fd_s) <= 1e-3 * abs(fd_s)

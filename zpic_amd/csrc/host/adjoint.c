@@ -1,0 +1,426 @@
+/*
+ * adjoint.c - differentiable runs: the loss of a run and its gradient with
+ * respect to the initial state, by the adjoint method with checkpoints.
+ *
+ * No counterpart in the reference.  The forward run is the ordinary time
+ * step (sim_iter, em2d/simulation.c:45-56) on the GPU; every `ckpt` steps
+ * the particles are tile-sorted and the state (particles with their ids,
+ * tile ranges, E, B) is copied aside.  The backward pass walks the
+ * segments from the last: it restores the segment's checkpoint, re-runs
+ * its steps while recording the state at the start of each, then applies
+ * the reverse step (adjoint.hip) from the last step of the segment to the
+ * first.  Particles keep their slots inside a segment; between segments
+ * the particle adjoints are carried across the checkpoint's sort through
+ * the ids.
+ *
+ * The loss is  L = sum over steps n in [loss_first, loss_last] of
+ * dt * (magnetic terms of emf_get_energy at step n), FP64; its gradient
+ * with respect to the grid B is dt * dx * dy * B on interior nodes.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../kernels/zk_adjoint.h"
+#include "zpic_internal.h"
+
+typedef struct {
+    zk_parts p;     /* particles (with ids) */
+    int *tile_off;  /* tile ranges */
+    zk_spec_ctr ctr;
+} spec_ckpt;
+
+struct zpic_diff {
+    t_simulation *sim;
+    int nsteps, ckpt, nck, loss_first, loss_last;
+    int ns;
+    size_t nodes;
+    int *np;           /* particles per species */
+    spec_ckpt *cs;     /* [(nck + 1) * ns]: checkpoints, then the final state */
+    float **cE, **cB;  /* [nck + 1] */
+    zk_parts *rp;      /* [ckpt * ns]: particle state at the start of each step of a segment */
+    float **rE, **rB;  /* [ckpt + 1]: fields at the start of each step, then at the end */
+    zk_adj_vec *lam, *lam2; /* per species */
+    int **pos;
+    float *lE, *lB, *lJ, *aE, *aB, *Lam, *region;
+    int *n_listed;
+    float *out;        /* device staging of one species' adjoints by id */
+    int forward_done;
+    int iter0;
+};
+
+#define CS(D, c, k) (&(D)->cs[(size_t)(c) * (D)->ns + (k)])
+
+static void parts_alloc(zk_parts *p, int n, int ids)
+{
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    p->ix = zpic_dmalloc(4 * m);
+    p->iy = zpic_dmalloc(4 * m);
+    p->x = zpic_dmalloc(4 * m);
+    p->y = zpic_dmalloc(4 * m);
+    p->ux = zpic_dmalloc(4 * m);
+    p->uy = zpic_dmalloc(4 * m);
+    p->uz = zpic_dmalloc(4 * m);
+    p->id = ids ? zpic_dmalloc(8 * m) : NULL;
+}
+
+static void parts_free(zk_parts *p)
+{
+    zpic_dfree(p->ix);
+    zpic_dfree(p->iy);
+    zpic_dfree(p->x);
+    zpic_dfree(p->y);
+    zpic_dfree(p->ux);
+    zpic_dfree(p->uy);
+    zpic_dfree(p->uz);
+    zpic_dfree(p->id);
+    memset(p, 0, sizeof(*p));
+}
+
+static void parts_copy(const zk_parts *dst, const zk_parts *src, int n)
+{
+    const size_t b = 4 * (size_t)n;
+    zpic_d2d_async(dst->ix, src->ix, b);
+    zpic_d2d_async(dst->iy, src->iy, b);
+    zpic_d2d_async(dst->x, src->x, b);
+    zpic_d2d_async(dst->y, src->y, b);
+    zpic_d2d_async(dst->ux, src->ux, b);
+    zpic_d2d_async(dst->uy, src->uy, b);
+    zpic_d2d_async(dst->uz, src->uz, b);
+    if (dst->id && src->id) zpic_d2d_async(dst->id, src->id, 2 * b);
+}
+
+static void vec_alloc(zk_adj_vec *v, int n)
+{
+    const size_t m = 4 * (size_t)(n > 0 ? n : 1);
+    v->lx = zpic_dmalloc(m);
+    v->ly = zpic_dmalloc(m);
+    v->lux = zpic_dmalloc(m);
+    v->luy = zpic_dmalloc(m);
+    v->luz = zpic_dmalloc(m);
+}
+
+static void vec_free(zk_adj_vec *v)
+{
+    zpic_dfree(v->lx);
+    zpic_dfree(v->ly);
+    zpic_dfree(v->lux);
+    zpic_dfree(v->luy);
+    zpic_dfree(v->luz);
+}
+
+static void vec_zero(const zk_adj_vec *v, int n)
+{
+    const zk_stream_t s = zpic_stream();
+    ZCHECK(zk_fill_f32(v->lx, (size_t)n, 0.0f, s));
+    ZCHECK(zk_fill_f32(v->ly, (size_t)n, 0.0f, s));
+    ZCHECK(zk_fill_f32(v->lux, (size_t)n, 0.0f, s));
+    ZCHECK(zk_fill_f32(v->luy, (size_t)n, 0.0f, s));
+    ZCHECK(zk_fill_f32(v->luz, (size_t)n, 0.0f, s));
+}
+
+zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first, int loss_last)
+{
+    if (!sim || !sim->emf.dev) zpic_fatal("zpic_diff_new: simulation was not created with sim_new");
+    if (nsteps <= 0 || ckpt <= 0) zpic_fatal("zpic_diff_new: steps and checkpoint interval must be positive");
+    if (sim->moving_window || sim->emf.moving_window)
+        zpic_fatal("zpic_diff_new: differentiable runs do not support a moving window");
+    if (sim->emf.ext_fld.E_type != EMF_FLD_TYPE_NONE || sim->emf.ext_fld.B_type != EMF_FLD_TYPE_NONE)
+        zpic_fatal("zpic_diff_new: differentiable runs do not support external fields");
+    if (sim->current.smooth.xtype != NONE || sim->current.smooth.ytype != NONE)
+        zpic_fatal("zpic_diff_new: differentiable runs do not support current filtering");
+    zpic_diff *D = calloc(1, sizeof(*D));
+    D->sim = sim;
+    D->nsteps = nsteps;
+    D->ckpt = ckpt;
+    D->nck = (nsteps + ckpt - 1) / ckpt;
+    D->loss_first = loss_first;
+    D->loss_last = loss_last;
+    D->ns = sim->n_species;
+    D->nodes = zpic_grid_nodes(sim->emf.nx);
+    D->np = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(int));
+    for (int k = 0; k < D->ns; k++) {
+        t_species *s = &sim->species[k];
+        if (s->moving_window) zpic_fatal("zpic_diff_new: differentiable runs do not support a moving window");
+        spec_track_ids(s);
+        zpic_species_ensure(s);
+        zpic_current_ensure(&sim->current, &s->dev->t);
+        zpic_spec_sort(s);
+        zk_spec_ctr c;
+        zpic_d2h(&c, s->dev->ctr, sizeof(c));
+        if (c.n_dead || c.n_sorted != c.np_total) zpic_fatal("zpic_diff_new: species '%s' is not sorted", s->name);
+        D->np[k] = c.np_total;
+    }
+    zpic_emf_ensure(&sim->emf);
+    zpic_current_ensure(&sim->current, NULL);
+    const size_t gb = sizeof(float) * 3 * D->nodes;
+    const zk_tiles *t = &sim->current.dev->t;
+
+    D->cs = calloc((size_t)(D->nck + 1) * (D->ns > 0 ? D->ns : 1), sizeof(spec_ckpt));
+    D->cE = calloc((size_t)D->nck + 1, sizeof(float *));
+    D->cB = calloc((size_t)D->nck + 1, sizeof(float *));
+    for (int c = 0; c <= D->nck; c++) {
+        D->cE[c] = zpic_dmalloc(gb);
+        D->cB[c] = zpic_dmalloc(gb);
+        for (int k = 0; k < D->ns; k++) {
+            parts_alloc(&CS(D, c, k)->p, D->np[k], 1);
+            CS(D, c, k)->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(t->ntiles + 2));
+        }
+    }
+    D->rp = calloc((size_t)D->ckpt * (D->ns > 0 ? D->ns : 1), sizeof(zk_parts));
+    for (int n = 0; n < D->ckpt; n++)
+        for (int k = 0; k < D->ns; k++) parts_alloc(&D->rp[(size_t)n * D->ns + k], D->np[k], 0);
+    D->rE = calloc((size_t)D->ckpt + 1, sizeof(float *));
+    D->rB = calloc((size_t)D->ckpt + 1, sizeof(float *));
+    for (int n = 0; n <= D->ckpt; n++) {
+        D->rE[n] = zpic_dmalloc(gb);
+        D->rB[n] = zpic_dmalloc(gb);
+    }
+    D->lam = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(zk_adj_vec));
+    D->lam2 = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(zk_adj_vec));
+    D->pos = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(int *));
+    int npmax = 1;
+    for (int k = 0; k < D->ns; k++) {
+        vec_alloc(&D->lam[k], D->np[k]);
+        vec_alloc(&D->lam2[k], D->np[k]);
+        D->pos[k] = zpic_dmalloc(sizeof(int) * (size_t)(D->np[k] > 0 ? D->np[k] : 1));
+        if (D->np[k] > npmax) npmax = D->np[k];
+    }
+    D->lE = zpic_dmalloc(gb);
+    D->lB = zpic_dmalloc(gb);
+    D->lJ = zpic_dmalloc(gb);
+    D->aE = zpic_dmalloc(gb);
+    D->aB = zpic_dmalloc(gb);
+    D->Lam = zpic_dmalloc(gb);
+    D->region = zpic_dmalloc(sizeof(float) * 6 * (size_t)t->WX * t->WY * t->ntiles);
+    D->n_listed = zpic_dmalloc(sizeof(int));
+    const int zero = 0;
+    zpic_h2d(D->n_listed, &zero, sizeof(int));
+    D->out = zpic_dmalloc(sizeof(float) * 5 * (size_t)npmax);
+    return D;
+}
+
+/* copy the simulation state into slot c (sorting the particles first) */
+static void save_state(zpic_diff *D, int c)
+{
+    t_simulation *sim = D->sim;
+    for (int k = 0; k < D->ns; k++) {
+        t_species *s = &sim->species[k];
+        struct zpic_dev_species *d = s->dev;
+        zpic_spec_sort(s);
+        spec_ckpt *C = CS(D, c, k);
+        parts_copy(&C->p, &d->set[d->cur], D->np[k]);
+        zpic_d2d_async(C->tile_off, d->tile_off, sizeof(int) * (size_t)(d->t.ntiles + 2));
+        zpic_d2h(&C->ctr, d->ctr, sizeof(zk_spec_ctr));
+        if (C->ctr.np_total != D->np[k] || C->ctr.n_sorted != C->ctr.np_total || C->ctr.n_dead)
+            zpic_fatal("zpic_diff: species '%s' changed its particle count", s->name);
+    }
+    struct zpic_dev_emf *de = sim->emf.dev;
+    const size_t gb = sizeof(float) * 3 * D->nodes;
+    zpic_d2d_async(D->cE[c], de->E[de->cur], gb);
+    zpic_d2d_async(D->cB[c], de->B[de->cur], gb);
+}
+
+/* put slot c back into the simulation, at iteration `iter` */
+static void restore_state(zpic_diff *D, int c, int iter)
+{
+    t_simulation *sim = D->sim;
+    for (int k = 0; k < D->ns; k++) {
+        t_species *s = &sim->species[k];
+        struct zpic_dev_species *d = s->dev;
+        spec_ckpt *C = CS(D, c, k);
+        parts_copy(&d->set[d->cur], &C->p, D->np[k]);
+        zpic_d2d_async(d->tile_off, C->tile_off, sizeof(int) * (size_t)(d->t.ntiles + 2));
+        zpic_h2d(d->ctr, &C->ctr, sizeof(zk_spec_ctr));
+        d->host_valid = 0;
+        s->iter = iter;
+    }
+    struct zpic_dev_emf *de = sim->emf.dev;
+    const size_t gb = sizeof(float) * 3 * D->nodes;
+    zpic_d2d_async(de->E[de->cur], D->cE[c], gb);
+    zpic_d2d_async(de->B[de->cur], D->cB[c], gb);
+    de->host_valid = 0;
+    sim->emf.iter = iter;
+    sim->current.iter = iter;
+    sim->current.dev->host_valid = 0;
+}
+
+static int in_loss(const zpic_diff *D, int n) { return n >= D->loss_first && n <= D->loss_last; }
+
+static double b_energy(t_simulation *sim)
+{
+    double e[6];
+    emf_get_energy(&sim->emf, e);
+    return e[3] + e[4] + e[5];
+}
+
+/* sim_iter with the species' sorts switched off (the checkpoints sort) */
+static void step_unsorted(zpic_diff *D)
+{
+    t_simulation *sim = D->sim;
+    int saved[64];
+    const int ns = D->ns < 64 ? D->ns : 64;
+    for (int k = 0; k < ns; k++) {
+        saved[k] = sim->species[k].n_sort;
+        sim->species[k].n_sort = 0;
+    }
+    sim_iter(sim);
+    for (int k = 0; k < ns; k++) sim->species[k].n_sort = saved[k];
+}
+
+double zpic_diff_forward(zpic_diff *D)
+{
+    if (!D) zpic_fatal("zpic_diff_forward: no run");
+    if (D->ns > 64) zpic_fatal("zpic_diff_forward: at most 64 species");
+    t_simulation *sim = D->sim;
+    const double dt = sim->dt;
+    D->iter0 = sim->emf.iter;
+    double loss = 0.0;
+    if (in_loss(D, 0)) loss += dt * b_energy(sim);
+    for (int n = 0; n < D->nsteps; n++) {
+        if (n % D->ckpt == 0) save_state(D, n / D->ckpt);
+        step_unsorted(D);
+        if (in_loss(D, n + 1)) loss += dt * b_energy(sim);
+    }
+    save_state(D, D->nck);
+    D->forward_done = 1;
+    return loss;
+}
+
+void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
+{
+    if (!D || !D->forward_done) zpic_fatal("zpic_diff_backward: run zpic_diff_forward first");
+    t_simulation *sim = D->sim;
+    t_emf *emf = &sim->emf;
+    const zk_stream_t st = zpic_stream();
+    const size_t gb = sizeof(float) * 3 * D->nodes;
+    const int nx = emf->nx[0], ny = emf->nx[1], nrow = emf->nrow;
+    const float scale = (float)((double)sim->dt * emf->dx[0] * emf->dx[1]);
+    ZCHECK(zk_fill_f32(D->lE, 3 * D->nodes, 0.0f, st));
+    ZCHECK(zk_fill_f32(D->lB, 3 * D->nodes, 0.0f, st));
+    for (int k = 0; k < D->ns; k++) vec_zero(&D->lam[k], D->np[k]);
+    struct zpic_dev_emf *de = emf->dev;
+
+    for (int c = D->nck - 1; c >= 0; c--) {
+        const int s0 = c * D->ckpt, s1 = (s0 + D->ckpt < D->nsteps) ? s0 + D->ckpt : D->nsteps;
+        /* re-run the segment, recording the state at the start of each step */
+        restore_state(D, c, D->iter0 + s0);
+        for (int n = s0; n < s1; n++) {
+            for (int k = 0; k < D->ns; k++) {
+                struct zpic_dev_species *d = sim->species[k].dev;
+                parts_copy(&D->rp[(size_t)(n - s0) * D->ns + k], &d->set[d->cur], D->np[k]);
+            }
+            zpic_d2d_async(D->rE[n - s0], de->E[de->cur], gb);
+            zpic_d2d_async(D->rB[n - s0], de->B[de->cur], gb);
+            step_unsorted(D);
+        }
+        zpic_d2d_async(D->rE[s1 - s0], de->E[de->cur], gb);
+        zpic_d2d_async(D->rB[s1 - s0], de->B[de->cur], gb);
+        /* particle adjoints: from the order of the next checkpoint (sorted
+         * after this segment) to this segment's order */
+        if (c < D->nck - 1) {
+            for (int k = 0; k < D->ns; k++) {
+                ZCHECK(zk_adj_permute(&D->lam[k], &D->lam2[k], CS(D, c + 1, k)->p.id, CS(D, c, k)->p.id, D->pos[k],
+                                      D->np[k], st));
+                const zk_adj_vec tmp = D->lam[k];
+                D->lam[k] = D->lam2[k];
+                D->lam2[k] = tmp;
+            }
+        }
+        for (int n = s1 - 1; n >= s0; n--) {
+            if (in_loss(D, n + 1)) ZCHECK(zk_adj_loss(D->lB, D->rB[n + 1 - s0], nx, ny, nrow, scale, st));
+            ZCHECK(zk_adj_fields(D->lE, D->lB, D->lJ, D->aE, D->aB, nx, ny, nrow, sim->dt, emf->dx[0], emf->dx[1],
+                                 st));
+            ZCHECK(zk_adj_images(D->lJ, D->Lam, nx, ny, nrow, st));
+            for (int k = 0; k < D->ns; k++) {
+                t_species *s = &sim->species[k];
+                struct zpic_dev_species *d = s->dev;
+                zk_adj_push_args a;
+                memset(&a, 0, sizeof(a));
+                a.p = D->rp[(size_t)(n - s0) * D->ns + k];
+                a.tile_off = CS(D, c, k)->tile_off;
+                a.E = D->rE[n - s0];
+                a.B = D->rB[n - s0];
+                a.lam = D->Lam;
+                a.nrow = nrow;
+                a.lx = D->lam[k].lx;
+                a.ly = D->lam[k].ly;
+                a.lux = D->lam[k].lux;
+                a.luy = D->lam[k].luy;
+                a.luz = D->lam[k].luz;
+                a.region = D->region;
+                a.lE = D->lE;
+                a.lB = D->lB;
+                a.t = d->t;
+                a.ovf_list = d->ovf_list;
+                a.n_listed = D->n_listed;
+                a.global_blocks = ZPIC_GLOBAL_BLOCKS;
+                /* the push constants of spec_advance */
+                a.tem = 0.5 * s->dt / s->m_q;
+                a.dt_dx = s->dt / s->dx[0];
+                a.dt_dy = s->dt / s->dx[1];
+                a.qnx = s->q * s->dx[0] / s->dt;
+                a.qny = s->q * s->dx[1] / s->dt;
+                a.q = s->q;
+                ZCHECK(zk_adj_push(&a, st));
+            }
+        }
+    }
+    if (in_loss(D, 0)) ZCHECK(zk_adj_loss(D->lB, D->cB[0], nx, ny, nrow, scale, st));
+
+    if (dE) zpic_d2h(dE, D->lE, gb);
+    if (dB) zpic_d2h(dB, D->lB, gb);
+    for (int k = 0; k < D->ns; k++) {
+        if (!dpart || !dpart[k]) continue;
+        ZCHECK(zk_adj_by_id(&D->lam[k], CS(D, 0, k)->p.id, D->np[k], D->out, st));
+        zpic_d2h(dpart[k], D->out, sizeof(float) * 5 * (size_t)D->np[k]);
+    }
+    /* leave the simulation in its state after the forward run */
+    restore_state(D, D->nck, D->iter0 + D->nsteps);
+    zpic_sync();
+}
+
+void zpic_diff_delete(zpic_diff *D)
+{
+    if (!D) return;
+    zpic_sync();
+    for (int c = 0; c <= D->nck; c++) {
+        zpic_dfree(D->cE[c]);
+        zpic_dfree(D->cB[c]);
+        for (int k = 0; k < D->ns; k++) {
+            parts_free(&CS(D, c, k)->p);
+            zpic_dfree(CS(D, c, k)->tile_off);
+        }
+    }
+    for (int n = 0; n < D->ckpt; n++)
+        for (int k = 0; k < D->ns; k++) parts_free(&D->rp[(size_t)n * D->ns + k]);
+    for (int n = 0; n <= D->ckpt; n++) {
+        zpic_dfree(D->rE[n]);
+        zpic_dfree(D->rB[n]);
+    }
+    for (int k = 0; k < D->ns; k++) {
+        vec_free(&D->lam[k]);
+        vec_free(&D->lam2[k]);
+        zpic_dfree(D->pos[k]);
+    }
+    zpic_dfree(D->lE);
+    zpic_dfree(D->lB);
+    zpic_dfree(D->lJ);
+    zpic_dfree(D->aE);
+    zpic_dfree(D->aB);
+    zpic_dfree(D->Lam);
+    zpic_dfree(D->region);
+    zpic_dfree(D->n_listed);
+    zpic_dfree(D->out);
+    free(D->cs);
+    free(D->cE);
+    free(D->cB);
+    free(D->rp);
+    free(D->rE);
+    free(D->rB);
+    free(D->lam);
+    free(D->lam2);
+    free(D->pos);
+    free(D->np);
+    free(D);
+}

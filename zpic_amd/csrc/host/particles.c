@@ -420,6 +420,8 @@ static void sort_now(t_species *s, int spread)
     d->moved_since_sort = 0;
 }
 
+void zpic_spec_sort(t_species *s) { sort_now(s, 0); }
+
 /* Upload the host mirror into the device buffers and sort it into tiles. */
 static void upload(t_species *s)
 {

@@ -80,6 +80,7 @@ const float *zpic_emf_particle_B(t_emf *emf);
 const float *zpic_current_J(t_current *cur);
 
 void zpic_spec_sync_energy(t_species *s);
+void zpic_spec_sort(t_species *s); /* tile sort of the device particles now */
 int zpic_current_passes(const t_current *c, int axis[], float sa[], float sb[], int max);
 
 #endif
