@@ -179,6 +179,63 @@ def batch_workload(args) -> dict:
     }
 
 
+def adjoint_workload(args) -> dict:
+    """Differentiable 256^2 Weibel run (zpic_amd.diff defaults: 2 electron
+    species of 4x4 ppc, 200 steps, checkpoints every 20, loss over steps
+    150-200, 64x64 B_z seed): time of the checkpointed forward run and of
+    the backward pass (re-run of every segment + reverse steps)."""
+    import numpy as np
+
+    from zpic_amd import _lib
+    from zpic_amd.diff import WeibelConfig, WeibelRun
+
+    lib = _lib.load()
+    cfg = WeibelConfig()
+    seed = 1e-3 * np.random.default_rng(0).standard_normal((cfg.ny // cfg.block, cfg.nx // cfg.block))
+    # warm-up run (kernel loading, allocations)
+    w = WeibelRun(WeibelConfig(nx=64, ny=64, steps=20, ckpt=10, loss_first=10, loss_last=20), 0.6, 0.1,
+                  np.zeros((16, 16)))
+    w.forward()
+    w.gradients()
+    w.close()
+    log("adjoint: building the 256x256 run")
+    run = WeibelRun(cfg, 0.6, 0.1, seed)
+    try:
+        lib.zpic_synchronize()
+        t0 = time.perf_counter()
+        loss = run.forward()
+        lib.zpic_synchronize()
+        t_fwd = time.perf_counter() - t0
+        log(f"adjoint: forward {t_fwd:.3f} s; backward")
+        t0 = time.perf_counter()
+        g_uz, g_uth, g_seed = run.gradients()
+        t_bwd = time.perf_counter() - t0
+        npart = sum(run.np)
+    finally:
+        run.close()
+    return {
+        "metric": "adjoint/fwd",
+        "value": t_bwd / t_fwd,
+        "unit": "ratio of backward to forward time",
+        "n_gpus": 1,
+        "steps": cfg.steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * t_bwd / cfg.steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Weibel initial state, seeded 64x64 B_z perturbation (amplitude 1e-3)",
+        "config": {"workload": (f"differentiable Weibel {cfg.nx}x{cfg.ny}, 2 electron species {cfg.ppc}x{cfg.ppc} "
+                                f"ppc ({npart} particles), {cfg.steps} steps, checkpoints every {cfg.ckpt}, loss "
+                                f"= sum of dt * B energy over steps {cfg.loss_first}-{cfg.loss_last}")},
+        "forward_ms_per_step": 1e3 * t_fwd / cfg.steps,
+        "backward_ms_per_step": 1e3 * t_bwd / cfg.steps,
+        "loss": loss,
+        "grad": {"uz": g_uz, "uth": g_uth, "seed_norm": float(np.linalg.norm(g_seed))},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,8 +251,9 @@ def main() -> int:
     ap.add_argument("--cpu-nx", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
-    ap.add_argument("--workload", choices=["weibel", "batch"], default="weibel",
-                    help="weibel: the 2048^2 Weibel step (default); batch: the batched parameter scan")
+    ap.add_argument("--workload", choices=["weibel", "batch", "adjoint"], default="weibel",
+                    help="weibel: the 2048^2 Weibel step (default); batch: the batched parameter scan; "
+                         "adjoint: backward/forward time of the differentiable 256^2 Weibel run")
     ap.add_argument("--members", type=int, default=512, help="batch: members of the scan")
     ap.add_argument("--batch-steps", type=int, default=300, help="batch: steps per member")
     ap.add_argument("--seq-stride", type=int, default=1, help="batch: run every n-th member sequentially")
@@ -216,6 +274,9 @@ def main() -> int:
     ensure_built()
     if args.workload == "batch":
         print(json.dumps(batch_workload(args)), flush=True)
+        return 0
+    if args.workload == "adjoint":
+        print(json.dumps(adjoint_workload(args)), flush=True)
         return 0
     import ctypes as C
 

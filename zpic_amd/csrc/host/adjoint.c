@@ -43,7 +43,7 @@ struct zpic_diff {
     zk_adj_vec *lam, *lam2; /* per species */
     int **pos;
     float *lE, *lB, *lJ, *aE, *aB, *Lam, *region;
-    int *n_listed;
+    int *ovf_cnt;
     float *out;        /* device staging of one species' adjoints by id */
     int forward_done;
     int iter0;
@@ -193,9 +193,7 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
     D->aB = zpic_dmalloc(gb);
     D->Lam = zpic_dmalloc(gb);
     D->region = zpic_dmalloc(sizeof(float) * 6 * (size_t)t->WX * t->WY * t->ntiles);
-    D->n_listed = zpic_dmalloc(sizeof(int));
-    const int zero = 0;
-    zpic_h2d(D->n_listed, &zero, sizeof(int));
+    D->ovf_cnt = zpic_dmalloc(sizeof(int) * (size_t)(t->ntiles > 0 ? t->ntiles : 1));
     D->out = zpic_dmalloc(sizeof(float) * 5 * (size_t)npmax);
     return D;
 }
@@ -353,7 +351,7 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
                 a.lB = D->lB;
                 a.t = d->t;
                 a.ovf_list = d->ovf_list;
-                a.n_listed = D->n_listed;
+                a.ovf_cnt = D->ovf_cnt;
                 a.global_blocks = ZPIC_GLOBAL_BLOCKS;
                 /* the push constants of spec_advance */
                 a.tem = 0.5 * s->dt / s->m_q;
@@ -410,7 +408,7 @@ void zpic_diff_delete(zpic_diff *D)
     zpic_dfree(D->aB);
     zpic_dfree(D->Lam);
     zpic_dfree(D->region);
-    zpic_dfree(D->n_listed);
+    zpic_dfree(D->ovf_cnt);
     zpic_dfree(D->out);
     free(D->cs);
     free(D->cE);

@@ -289,6 +289,8 @@ static void species_free_device(struct zpic_dev_species *d)
     d->ovf_list = NULL;
     zpic_dfree(d->tile_off);
     zpic_dfree(d->cnt);
+    zpic_dfree(d->ovf_cnt);
+    d->ovf_cnt = NULL;
     zpic_dfree(d->cursor);
     zpic_dfree(d->epart);
     zpic_dfree(d->stage);
@@ -471,6 +473,7 @@ void zpic_species_ensure(t_species *s)
         zpic_h2d(d->ctr, &zero, sizeof(zero));
         d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->ovf_cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         d->cursor = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         ZCHECK(zk_fill_f32((float *)d->cnt, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */
         d->epart_cap = d->t.ntiles + ZPIC_GLOBAL_BLOCKS;
@@ -621,6 +624,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     /* the global pass takes the listed out-of-window particles and the tail */
     const int tb = ZPIC_GLOBAL_BLOCKS;
     a.ovf_list = d->ovf_list;
+    a.ovf_cnt = d->ovf_cnt;
     a.global_blocks = tb;
     a.waves = zpic_push_waves();
     zpic_ktimer_begin(ZPIC_T_PUSH);

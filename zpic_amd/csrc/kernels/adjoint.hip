@@ -605,6 +605,8 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
     const int n = t.WX * t.WY;
     const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H;
     float *fl = sm, *ll = sm + 6 * n, *acc = sm + 9 * n;
+    __shared__ int nlist;
+    if (threadIdx.x == 0) nlist = 0;
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
         const int gx = ox + m % t.WX, gy = oy + m / t.WX;
         const bool in = gx >= -1 && gx <= t.nx + 1 && gy >= -1 && gy <= t.ny + 1;
@@ -628,9 +630,9 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
         if (out) {
             int base = 0;
             const int leader = __ffsll((long long)out) - 1;
-            if (lane == leader) base = atomicAdd(a.n_listed, __popcll(out));
+            if (lane == leader) base = atomicAdd(&nlist, __popcll(out));
             base = __shfl(base, leader, ZK_WAVE);
-            if (!inside) a.ovf_list[base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+            if (!inside) a.ovf_list[i0 + base + __popcll(out & ((1ull << lane) - 1ull))] = i;
         }
         if (!inside) continue;
         float l[5];
@@ -641,23 +643,25 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
     __syncthreads();
     float *dst = a.region + (size_t)tile * 6 * n;
     for (int m = threadIdx.x; m < 6 * n; m += blockDim.x) dst[m] = acc[m];
+    if (threadIdx.x == 0) a.ovf_cnt[tile] = nlist;
 }
 
 __global__ void __launch_bounds__(256) adj_global_kernel(zk_adj_push_args a)
 {
     const AdjConsts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
     const GlobalSrc src = {a.E, a.B, a.lam, a.lE, a.lB, a.nrow};
-    const int listed = *a.n_listed;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < listed; q += gridDim.x * blockDim.x) {
-        const int i = a.ovf_list[q];
-        float l[5];
-        load_adj(a, i, l);
-        adjoint_one(src, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
-        store_adj(a, i, l);
+    for (int tile = blockIdx.x; tile < a.t.ntiles; tile += gridDim.x) {
+        const int base = a.tile_off[tile], cnt = a.ovf_cnt[tile];
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            const int i = a.ovf_list[base + q];
+            float l[5];
+            load_adj(a, i, l);
+            adjoint_one(src, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l,
+                        k);
+            store_adj(a, i, l);
+        }
     }
 }
-
-__global__ void adj_reset_kernel(int *n_listed) { *n_listed = 0; }
 
 /* field adjoints += the tile windows covering each buffer node */
 __global__ void adj_reduce_kernel(const float *__restrict__ region, float *__restrict__ lE, float *__restrict__ lB,
@@ -766,7 +770,6 @@ extern "C" int zk_adj_push(const zk_adj_push_args *a, zk_stream_t s)
     }
     if (a->t.ntiles > 0) hipLaunchKernelGGL(adj_tile_kernel, dim3(a->t.ntiles), dim3(256), lds, s, *a);
     hipLaunchKernelGGL(adj_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
-    hipLaunchKernelGGL(adj_reset_kernel, dim3(1), dim3(1), 0, s, a->n_listed);
     const int nodes = (a->t.nx + 3) * (a->t.ny + 3);
     hipLaunchKernelGGL(adj_reduce_kernel, dim3(grid_of((size_t)nodes)), dim3(256), 0, s, a->region, a->lE, a->lB,
                        a->t, a->nrow);

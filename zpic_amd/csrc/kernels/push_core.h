@@ -283,13 +283,13 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
 {
     extern __shared__ float sm[];
     __shared__ double red[16];
-    __shared__ int ndead_blk;
+    __shared__ int ndead_blk, nlist;
     const zk_tiles &t = a.t;
     const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
     zk_parts P = a.p;
     double kin = 0.0;
     int ndead = 0;
-    if (threadIdx.x == 0) ndead_blk = 0;
+    if (threadIdx.x == 0) ndead_blk = nlist = 0;
 
     const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
     const int n = t.WX * t.WY;
@@ -322,14 +322,15 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
         const int lx = c.ix - tx0, ly = c.iy - ty0;
         const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
         const bool live = c.ix != ZK_DEAD;
-        /* list the live particles outside the window, one atomic per wave */
+        /* list the live particles outside the window in the tile's own
+         * range of ovf_list (one LDS atomic per wave) */
         const unsigned long long out = __ballot(live && !inside);
         if (out) {
             int base = 0;
             const int leader = __ffsll((long long)out) - 1;
-            if (lane == leader) base = atomicAdd(&a.ctr->n_listed, __popcll(out));
+            if (lane == leader) base = atomicAdd(&nlist, __popcll(out));
             base = __shfl(base, leader, ZK_WAVE);
-            if (live && !inside) a.ovf_list[base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+            if (live && !inside) a.ovf_list[i0 + base + __popcll(out & ((1ull << lane) - 1ull))] = i;
         }
         if (!live || !inside) continue;
         int di, dj;
@@ -352,7 +353,10 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     if (threadIdx.x == 0) a.epart[tile] = s;
     if (ndead) atomicAdd(&ndead_blk, ndead);
     __syncthreads();
-    if (threadIdx.x == 0 && ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
+    if (threadIdx.x == 0) {
+        if (ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
+        a.ovf_cnt[tile] = nlist;
+    }
 }
 
 /* Global pass: the particles listed by the tile pass, then the unsorted
@@ -371,20 +375,35 @@ __device__ inline void push_global_body(const zk_push_args &a, const int block, 
     double kin = 0.0;
     int ndead = 0, nov = 0;
     if (threadIdx.x == 0) ndead_blk = nov_blk = 0;
-    const int listed = a.ctr->n_listed;
+    /* the particles each tile listed (tile t's list starts at its first
+     * slot), then the unsorted tail */
     const int tail0 = a.ctr->n_sorted;
-    const int total = listed + (a.ctr->np_total - tail0);
-    for (int q = block * blockDim.x + threadIdx.x; q < total; q += nblocks * blockDim.x) {
-        const int i = q < listed ? a.ovf_list[q] : tail0 + (q - listed);
-        Part c = load_part(P, i);
-        if (c.ix == ZK_DEAD) continue;
-        int di, dj;
-        nov++;
-        kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-        c.ix += di;
-        c.iy += dj;
-        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-        store_part(P, i, c);
+    const int ntail = a.ctr->np_total - tail0;
+    for (int tile = block; tile < t.ntiles + nblocks; tile += nblocks) {
+        int base, cnt, first;
+        if (tile < t.ntiles) {
+            base = a.tile_off[tile];
+            cnt = a.ovf_cnt[tile];
+            first = threadIdx.x;
+        } else {
+            /* the tail, shared by all blocks */
+            base = tail0;
+            cnt = ntail;
+            first = (tile - t.ntiles) * blockDim.x + threadIdx.x;
+        }
+        const int stride = tile < t.ntiles ? blockDim.x : nblocks * blockDim.x;
+        for (int q = first; q < cnt; q += stride) {
+            const int i = tile < t.ntiles ? a.ovf_list[base + q] : base + q;
+            Part c = load_part(P, i);
+            if (c.ix == ZK_DEAD) continue;
+            int di, dj;
+            nov++;
+            kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+            c.ix += di;
+            c.iy += dj;
+            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+            store_part(P, i, c);
+        }
     }
     const double s = zk_block_sum(kin, red);
     if (threadIdx.x == 0) a.epart[t.ntiles + block] = s;
@@ -414,7 +433,6 @@ __device__ inline void finalize_body(zk_spec_ctr *ctr, const double *epart, int 
         for (int i = 0; i < (int)blockDim.x; i++) tot += part[i];
         ctr->energy = qm * tot * dx0 * dx1;
         ctr->npush += (unsigned long long)(ctr->np_total - ctr->n_dead);
-        ctr->n_listed = 0;
     }
 }
 

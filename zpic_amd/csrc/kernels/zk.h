@@ -36,7 +36,6 @@ typedef struct zk_spec_ctr {
     int n_sorted;               /* end of the tile ranges                     */
     int n_dead;                 /* removed particles still occupying slots    */
     int n_overflow;             /* pushes on the global path, accumulated     */
-    int n_listed;               /* out-of-window particles listed this step   */
     unsigned long long npush;   /* accumulated particle pushes                */
     double energy;              /* kinetic energy of the last push            */
 } zk_spec_ctr;
@@ -69,7 +68,9 @@ typedef struct zk_push_args {
     zk_tiles t;
     int moving_window;          /* x absorbing + window shift           */
     int shift;                  /* 1 when the window moves this step    */
-    int *ovf_list;              /* slots of out-of-window particles     */
+    int *ovf_list;              /* slots of out-of-window particles: tile t's
+                                   list at [tile_off[t], tile_off[t] + ovf_cnt[t]) */
+    int *ovf_cnt;               /* [ntiles] listed per tile               */
     int global_blocks;          /* blocks of the global (overflow) pass */
     int waves;                  /* tile-pass occupancy variant: 0 or 8   */
     float tem, dt_dx, dt_dy, qnx, qny, q;

@@ -30,8 +30,8 @@ typedef struct zk_adj_push_args {
     float *region;              /* [ntiles][6][WY][WX] field-adjoint windows */
     float *lE, *lB;             /* field adjoints (accumulated) */
     zk_tiles t;
-    int *ovf_list;              /* slots outside their tile window */
-    int *n_listed;              /* device counter, zero on entry and exit */
+    int *ovf_list;              /* slots outside their tile window, tile t's at tile_off[t] */
+    int *ovf_cnt;               /* [ntiles] listed per tile */
     int global_blocks;
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_adj_push_args;
