@@ -112,7 +112,6 @@ static void fill_spec(zpic_batch *b, int k)
     a->moving_window = 0;
     a->shift = 0;
     a->ovf_list = d->ovf_list;
-    a->ovf_cnt = d->ovf_cnt;
     a->global_blocks = b->gblocks[k];
     a->waves = b->waves;
     a->tem = 0.5 * s->dt / s->m_q;

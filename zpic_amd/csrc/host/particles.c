@@ -289,8 +289,6 @@ static void species_free_device(struct zpic_dev_species *d)
     d->ovf_list = NULL;
     zpic_dfree(d->tile_off);
     zpic_dfree(d->cnt);
-    zpic_dfree(d->ovf_cnt);
-    d->ovf_cnt = NULL;
     zpic_dfree(d->cursor);
     zpic_dfree(d->epart);
     zpic_dfree(d->stage);
@@ -473,7 +471,6 @@ void zpic_species_ensure(t_species *s)
         zpic_h2d(d->ctr, &zero, sizeof(zero));
         d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
-        d->ovf_cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         d->cursor = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         ZCHECK(zk_fill_f32((float *)d->cnt, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */
         d->epart_cap = d->t.ntiles + ZPIC_GLOBAL_BLOCKS;
@@ -621,10 +618,9 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     const int next_iter = s->iter + 1;
     const int move = s->moving_window && ((next_iter * s->dt) > (s->dx[0] * (s->n_move + 1)));
     a.shift = move;
-    /* the global pass takes the listed out-of-window particles and the tail */
-    const int tb = ZPIC_GLOBAL_BLOCKS;
+    /* the global pass pushes the unsorted tail (injected particles), if any */
+    const int tb = d->tail_upper > 0 ? ZPIC_GLOBAL_BLOCKS : 0;
     a.ovf_list = d->ovf_list;
-    a.ovf_cnt = d->ovf_cnt;
     a.global_blocks = tb;
     a.waves = zpic_push_waves();
     zpic_ktimer_begin(ZPIC_T_PUSH);
@@ -639,7 +635,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
         d->moved_since_sort++;
     }
     const float qm = s->q * s->m_q;
-    ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + ZPIC_GLOBAL_BLOCKS, (double)qm, (double)s->dx[0],
+    ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0],
                             (double)s->dx[1],
                             zpic_stream()));
     if ((s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0) sort_now(s, 0);

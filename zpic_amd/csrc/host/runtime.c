@@ -24,7 +24,7 @@ void zpic_synchronize(void);
 static int g_tx = 16, g_ty = 16, g_halo = 3;
 static int g_push_waves = 0;
 
-void zpic_set_push_waves(int waves) { g_push_waves = (waves == 8) ? 8 : 0; }
+void zpic_set_push_waves(int waves) { g_push_waves = (waves == 8 || waves == 2) ? waves : 0; }
 int zpic_push_waves(void) { return g_push_waves; }
 
 void zpic_check(int status, const char *what, const char *file, int line)

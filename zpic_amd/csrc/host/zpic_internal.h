@@ -32,7 +32,6 @@ struct zpic_dev_species {
     zk_spec_ctr *ctr;     /* device counters */
     zk_tiles t;
     int *tile_off, *cnt, *cursor;
-    int *ovf_cnt;         /* out-of-window particles listed per tile */
     int *ovf_list;        /* slots listed by the tile pass for the global pass */
     double *epart;
     int epart_cap;

@@ -214,11 +214,13 @@ __device__ inline void store_part(const zk_parts &P, int i, const Part &c)
     __builtin_nontemporal_store(c.uz, &P.uz[i]);
 }
 
-/* Push one particle whose cell is (ci, cj) in the sources' index space;
- * returns its time-centred kinetic energy term (em2d/particles.c:1125-1228). */
-template <class F, class D>
-__device__ inline double push_one(const F &S, const D &J, int ci, int cj, float &x, float &y,
-                                  float &ux, float &uy, float &uz, int &di, int &dj, const Consts &k)
+/* Motion of one particle whose cell is (ci, cj) in the sources' index
+ * space: gather, Boris push, displacement (mx, my), cells crossed (di, dj)
+ * and q v_z; returns the time-centred kinetic energy term
+ * (em2d/particles.c:1125-1228). */
+template <class F>
+__device__ inline double move_one(const F &S, int ci, int cj, float x, float y, float &ux, float &uy, float &uz,
+                                  float &mx, float &my, float &qvz, int &di, int &dj, const Consts &k)
 {
     float e[3], b[3];
     gather(S, ci, cj, x, y, e, b);
@@ -248,14 +250,26 @@ __device__ inline double push_one(const F &S, const D &J, int ci, int cj, float 
     uz = az + e[2];
 
     const float rg = 1.0f / sqrtf(1.0f + ux * ux + uy * uy + uz * uz);
-    const float mx = k.dt_dx * rg * ux;
-    const float my = k.dt_dy * rg * uy;
-    float x1 = x + mx, y1 = y + my;
+    mx = k.dt_dx * rg * ux;
+    my = k.dt_dy * rg * uy;
+    const float x1 = x + mx, y1 = y + my;
     di = (x1 >= 1.0f) - (x1 < 0.0f);
     dj = (y1 >= 1.0f) - (y1 < 0.0f);
+    qvz = k.q * uz * rg;
+    return kin;
+}
+
+/* Push one particle: move_one, the deposit of its motion, and its new cell
+ * offsets (ltrim, em2d/particles.c:1081-1084, 1200-1212). */
+template <class F, class D>
+__device__ inline double push_one(const F &S, const D &J, int ci, int cj, float &x, float &y,
+                                  float &ux, float &uy, float &uz, int &di, int &dj, const Consts &k)
+{
+    float mx, my, qvz;
+    const double kin = move_one(S, ci, cj, x, y, ux, uy, uz, mx, my, qvz, di, dj, k);
+    float x1 = x + mx, y1 = y + my;
     x1 -= di;
     y1 -= dj;
-    const float qvz = k.q * uz * rg;
     deposit(J, ci, cj, di, dj, x, y, mx, my, k.qnx, k.qny, qvz);
     x = x1;
     y = y1;
@@ -279,6 +293,25 @@ __device__ inline bool boundaries(int &ix, int &iy, int nx, int ny, int mw, int 
     return false;
 }
 
+/* List the live particles outside the window in the tile's own range of
+ * ovf_list (one LDS atomic per wave). */
+__device__ inline void list_out(const zk_push_args &a, int i0, int i, bool out_here, int *nlist)
+{
+    const unsigned long long out = __ballot(out_here);
+    if (out) {
+        const int lane = __lane_id();
+        int base = 0;
+        const int leader = __ffsll((long long)out) - 1;
+        if (lane == leader) base = atomicAdd(nlist, __popcll(out));
+        base = __shfl(base, leader, ZK_WAVE);
+        if (out_here) a.ovf_list[i0 + base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+    }
+}
+
+/* Tile pass.  PAIR = 2 pushes two particles per thread and iteration: the
+ * gathers and Boris pushes of both are independent and interleave, which
+ * hides LDS and arithmetic latency at the price of registers. */
+template <int PAIR = 1>
 __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
 {
     extern __shared__ float sm[];
@@ -316,31 +349,88 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     const LdsFields lf = {fl, t.WX, n};
     const LdsCurrent lj = {jl, t.WX, n};
     const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
-    const int lane = __lane_id();
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        Part c = load_part(P, i);
-        const int lx = c.ix - tx0, ly = c.iy - ty0;
-        const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
-        const bool live = c.ix != ZK_DEAD;
-        /* list the live particles outside the window in the tile's own
-         * range of ovf_list (one LDS atomic per wave) */
-        const unsigned long long out = __ballot(live && !inside);
-        if (out) {
-            int base = 0;
-            const int leader = __ffsll((long long)out) - 1;
-            if (lane == leader) base = atomicAdd(&nlist, __popcll(out));
-            base = __shfl(base, leader, ZK_WAVE);
-            if (live && !inside) a.ovf_list[i0 + base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+    if (PAIR == 1) {
+        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            Part c = load_part(P, i);
+            const int lx = c.ix - tx0, ly = c.iy - ty0;
+            const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
+            const bool live = c.ix != ZK_DEAD;
+            list_out(a, i0, i, live && !inside, &nlist);
+            if (!live || !inside) continue;
+            int di, dj;
+            kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+            c.ix += di;
+            c.iy += dj;
+            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+            store_part(P, i, c);
         }
-        if (!live || !inside) continue;
-        int di, dj;
-        kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-        c.ix += di;
-        c.iy += dj;
-        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-        store_part(P, i, c);
+    } else {
+        for (int i = i0 + threadIdx.x; i < i1; i += 2 * blockDim.x) {
+            const int j = i + blockDim.x;
+            const bool hasb = j < i1;
+            Part ca = load_part(P, i);
+            Part cb = hasb ? load_part(P, j) : ca;
+            const int lxa = ca.ix - tx0, lya = ca.iy - ty0, lxb = cb.ix - tx0, lyb = cb.iy - ty0;
+            const bool ina = lxa >= -t.H && lxa < t.TX + t.H && lya >= -t.H && lya < t.TY + t.H;
+            const bool inb = lxb >= -t.H && lxb < t.TX + t.H && lyb >= -t.H && lyb < t.TY + t.H;
+            const bool livea = ca.ix != ZK_DEAD, liveb = hasb && cb.ix != ZK_DEAD;
+            list_out(a, i0, i, livea && !ina, &nlist);
+            list_out(a, i0, j, liveb && !inb, &nlist);
+            const bool pa = livea && ina, pb = liveb && inb;
+            /* a particle that is not pushed here moves at a safe cell */
+            const int cia = pa ? ca.ix - ox : t.H + 1, cja = pa ? ca.iy - oy : t.H + 1;
+            const int cib = pb ? cb.ix - ox : t.H + 1, cjb = pb ? cb.iy - oy : t.H + 1;
+            float mxa, mya, qa, mxb, myb, qb;
+            int dia, dja, dib, djb;
+            const double ka = move_one(lf, cia, cja, ca.x, ca.y, ca.ux, ca.uy, ca.uz, mxa, mya, qa, dia, dja, k);
+            const double kb = move_one(lf, cib, cjb, cb.x, cb.y, cb.ux, cb.uy, cb.uz, mxb, myb, qb, dib, djb, k);
+            if (pa) {
+                float x1 = ca.x + mxa, y1 = ca.y + mya;
+                x1 -= dia;
+                y1 -= dja;
+                deposit(lj, cia, cja, dia, dja, ca.x, ca.y, mxa, mya, k.qnx, k.qny, qa);
+                ca.x = x1;
+                ca.y = y1;
+                kin += ka;
+                ca.ix += dia;
+                ca.iy += dja;
+                if (boundaries(ca.ix, ca.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                store_part(P, i, ca);
+            }
+            if (pb) {
+                float x1 = cb.x + mxb, y1 = cb.y + myb;
+                x1 -= dib;
+                y1 -= djb;
+                deposit(lj, cib, cjb, dib, djb, cb.x, cb.y, mxb, myb, k.qnx, k.qny, qb);
+                cb.x = x1;
+                cb.y = y1;
+                kin += kb;
+                cb.ix += dib;
+                cb.iy += djb;
+                if (boundaries(cb.ix, cb.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                store_part(P, j, cb);
+            }
+        }
     }
     __syncthreads();
+    /* the tile's particles outside its window: global path, by this block
+     * (gathers from the guard-inclusive grids, atomics into Jov) */
+    const int nl = nlist;
+    if (nl > 0) {
+        const GlobalFields gf = {a.E, a.B, a.nrow};
+        const GlobalCurrent gj = {a.Jov, a.nrow};
+        for (int q = threadIdx.x; q < nl; q += blockDim.x) {
+            const int i = a.ovf_list[i0 + q];
+            Part c = load_part(P, i);
+            int di, dj;
+            kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+            c.ix += di;
+            c.iy += dj;
+            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+            store_part(P, i, c);
+        }
+        if (threadIdx.x == 0) atomicAdd(&a.ctr->n_overflow, nl);
+    }
     /* store the current window of this tile */
     float *dst = a.region + (size_t)tile * 3 * n;
     if (a.fresh) {
@@ -353,16 +443,12 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     if (threadIdx.x == 0) a.epart[tile] = s;
     if (ndead) atomicAdd(&ndead_blk, ndead);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
-        a.ovf_cnt[tile] = nlist;
-    }
+    if (threadIdx.x == 0 && ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
 }
 
-/* Global pass: the particles listed by the tile pass, then the unsorted
- * tail (injected since the last sort), with gathers from the guard-inclusive
- * field grids as the reference indexes them and atomics into the raw
- * overflow current. */
+/* Global pass: the unsorted tail (particles injected since the last sort),
+ * with gathers from the guard-inclusive field grids as the reference
+ * indexes them and atomics into the raw overflow current. */
 __device__ inline void push_global_body(const zk_push_args &a, const int block, const int nblocks)
 {
     __shared__ double red[16];
@@ -375,35 +461,20 @@ __device__ inline void push_global_body(const zk_push_args &a, const int block, 
     double kin = 0.0;
     int ndead = 0, nov = 0;
     if (threadIdx.x == 0) ndead_blk = nov_blk = 0;
-    /* the particles each tile listed (tile t's list starts at its first
-     * slot), then the unsorted tail */
+    /* the unsorted tail */
     const int tail0 = a.ctr->n_sorted;
     const int ntail = a.ctr->np_total - tail0;
-    for (int tile = block; tile < t.ntiles + nblocks; tile += nblocks) {
-        int base, cnt, first;
-        if (tile < t.ntiles) {
-            base = a.tile_off[tile];
-            cnt = a.ovf_cnt[tile];
-            first = threadIdx.x;
-        } else {
-            /* the tail, shared by all blocks */
-            base = tail0;
-            cnt = ntail;
-            first = (tile - t.ntiles) * blockDim.x + threadIdx.x;
-        }
-        const int stride = tile < t.ntiles ? blockDim.x : nblocks * blockDim.x;
-        for (int q = first; q < cnt; q += stride) {
-            const int i = tile < t.ntiles ? a.ovf_list[base + q] : base + q;
-            Part c = load_part(P, i);
-            if (c.ix == ZK_DEAD) continue;
-            int di, dj;
-            nov++;
-            kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-            c.ix += di;
-            c.iy += dj;
-            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-            store_part(P, i, c);
-        }
+    for (int q = block * blockDim.x + threadIdx.x; q < ntail; q += nblocks * blockDim.x) {
+        const int i = tail0 + q;
+        Part c = load_part(P, i);
+        if (c.ix == ZK_DEAD) continue;
+        int di, dj;
+        nov++;
+        kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+        c.ix += di;
+        c.iy += dj;
+        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+        store_part(P, i, c);
     }
     const double s = zk_block_sum(kin, red);
     if (threadIdx.x == 0) a.epart[t.ntiles + block] = s;

@@ -68,10 +68,9 @@ typedef struct zk_push_args {
     zk_tiles t;
     int moving_window;          /* x absorbing + window shift           */
     int shift;                  /* 1 when the window moves this step    */
-    int *ovf_list;              /* slots of out-of-window particles: tile t's
-                                   list at [tile_off[t], tile_off[t] + ovf_cnt[t]) */
-    int *ovf_cnt;               /* [ntiles] listed per tile               */
-    int global_blocks;          /* blocks of the global (overflow) pass */
+    int *ovf_list;              /* slots of out-of-window particles: tile t
+                                   lists its own at tile_off[t] onwards */
+    int global_blocks;          /* blocks of the global (tail) pass      */
     int waves;                  /* tile-pass occupancy variant: 0 or 8   */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
