@@ -288,6 +288,9 @@ static void species_free_device(struct zpic_dev_species *d)
     zpic_dfree(d->ovf_list);
     d->ovf_list = NULL;
     zpic_dfree(d->tile_off);
+    zpic_dfree(d->tile_off2);
+    zpic_dfree(d->stay);
+    d->tile_off2 = d->stay = NULL;
     zpic_dfree(d->cnt);
     zpic_dfree(d->cursor);
     zpic_dfree(d->epart);
@@ -412,8 +415,18 @@ static void sort_now(t_species *s, int spread)
     struct zpic_dev_species *d = s->dev;
     const int other = 1 - d->cur;
     zpic_ktimer_begin(ZPIC_T_SORT);
-    ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, &d->t, d->np_upper,
-                   spread, zpic_stream()));
+    if (spread) {
+        /* an unsorted upload: counting sort with cell interleaving */
+        ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, &d->t, d->np_upper,
+                       spread, zpic_stream()));
+    } else {
+        /* the particles are in tile ranges (plus a tail): keep the stayers */
+        ZCHECK(zk_sort_tiles(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->stay, d->tile_off, d->tile_off2,
+                             d->cursor, &d->t, d->tail_upper, zpic_stream()));
+        int *tmp = d->tile_off;
+        d->tile_off = d->tile_off2;
+        d->tile_off2 = tmp;
+    }
     zpic_ktimer_end(ZPIC_T_SORT);
     d->cur = other;
     d->tail_upper = 0;
@@ -470,6 +483,8 @@ void zpic_species_ensure(t_species *s)
         const zk_spec_ctr zero = {0};
         zpic_h2d(d->ctr, &zero, sizeof(zero));
         d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
+        d->tile_off2 = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
+        d->stay = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         d->cursor = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         ZCHECK(zk_fill_f32((float *)d->cnt, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */

@@ -17,6 +17,9 @@
  * workgroup), scatter (write every live particle to its tile range).  Count
  * and scatter reserve slots with one atomic per run of equal keys in a
  * wave, so a tile-ordered input costs about one atomic per wave.
+ * zk_sort_tiles is the variant for inputs that are already tile-sorted
+ * (every regular re-sort): particles that stay in their tile are written
+ * in order, contiguously, and only those that changed tile use atomics.
  */
 #include "sort_core.h"
 
@@ -44,6 +47,24 @@ __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, i
                                  zk_tiles t, int spread)
 {
     scatter_body(src, dst, n_old, cursor, off, t, spread, blockIdx.x, gridDim.x);
+}
+
+__global__ void count_stay_kernel(const int *__restrict__ ix, const int *__restrict__ iy, const int *old_off,
+                                  int *stay, int *move, zk_tiles t)
+{
+    count_stay_body(ix, iy, old_off, stay, move, t, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(1024) scan_stay_kernel(const int *stay, int *move, int *new_off, int *cursor,
+                                                         zk_spec_ctr *ctr, int ntiles)
+{
+    scan_stay_body(stay, move, new_off, cursor, ctr, ntiles);
+}
+
+__global__ void scatter_stay_kernel(zk_parts src, zk_parts dst, const int *old_off, const int *new_off,
+                                    int *cursor, zk_tiles t)
+{
+    scatter_stay_body(src, dst, old_off, new_off, cursor, t, blockIdx.x, gridDim.x);
 }
 
 /* t_part AoS (28 bytes: ix, iy, x, y, ux, uy, uz) <-> SoA */
@@ -110,6 +131,24 @@ extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ct
     hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, ctr, cnt, *t);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles);
     hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t, spread);
+    ZK_LAUNCH_CHECK();
+}
+
+/* Sort of a tile-sorted input (ranges old_off, plus an unsorted tail of
+ * at most tail_upper slots): one workgroup per source tile keeps the
+ * particles that stay in their tile in order at the front of its new
+ * range with coalesced writes; particles that changed tile (and the tail)
+ * are appended to their new tile. */
+extern "C" int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *move, int *stay,
+                             int *old_off, int *new_off, int *cursor, const zk_tiles *t, int tail_upper,
+                             zk_stream_t s)
+{
+    const unsigned tb = tail_upper > 0 ? grid_for(tail_upper, 1024, 4096) : 0;
+    const unsigned g = (unsigned)t->ntiles + tb;
+    hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, old_off + t->ntiles + 1);
+    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, old_off, stay, move, *t);
+    hipLaunchKernelGGL(scan_stay_kernel, dim3(1), dim3(1024), 0, s, stay, move, new_off, cursor, ctr, t->ntiles);
+    hipLaunchKernelGGL(scatter_stay_kernel, dim3(g), dim3(256), 0, s, *src, *dst, old_off, new_off, cursor, *t);
     ZK_LAUNCH_CHECK();
 }
 
