@@ -114,9 +114,18 @@ def batch_workload(args) -> dict:
     seeds = list(range(args.members))
     steps = args.batch_steps
     threads = args.threads or min(16, os.cpu_count() or 1)
+    tiles = None
+    t_tune = 0.0
+    if args.tune:
+        t0 = time.time()
+        classes = {scan.size_class(scan.member_params(s)) for s in seeds}
+        log(f"batch: tuning tiles for {len(classes)} size classes")
+        tiles = scan.tune_tiles(classes)
+        t_tune = time.time() - t0
+        log(f"batch: tiles {tiles} ({t_tune:.1f} s)")
     log(f"batch: building {len(seeds)} members on {threads} host threads")
     t0 = time.time()
-    members = scan.build_members(seeds, threads=threads)
+    members = scan.build_members(seeds, threads=threads, tiles=tiles)
     npart = sum(m.particles for m in members)
     log(f"batch: init {time.time() - t0:.1f} s, {npart} particles; uploading")
     t0 = time.time()
@@ -140,6 +149,8 @@ def batch_workload(args) -> dict:
     e_seq = []
     for s in seeds[::args.seq_stride]:
         m = scan.Member(lib, scan.member_params(s))
+        if tiles and scan.size_class(m.params) in tiles:
+            lib.zpic_sim_set_tiling(C.addressof(m.sim), *tiles[scan.size_class(m.params)])
         m.step(1)  # upload and first step outside the timed region, as for the batch
         lib.zpic_synchronize()
         t0 = time.perf_counter()
@@ -175,6 +186,8 @@ def batch_workload(args) -> dict:
         "sequential": {"value": seq_rate, "unit": "sims/s", "members_run": nseq,
                        "note": "same members, one after another with sim_iter on the GPU"},
         "speedup_vs_sequential": (len(seeds) / t_batch) / seq_rate,
+        "tiles": {f"{k[0]}x{k[1]}": list(v) for k, v in tiles.items()} if tiles else "default 16x16, halo 3",
+        "tuning_s": t_tune,
         "energy_rel_diff_max": max(rel) if rel else None,
     }
 
@@ -258,6 +271,7 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=300, help="batch: steps per member")
     ap.add_argument("--seq-stride", type=int, default=1, help="batch: run every n-th member sequentially")
     ap.add_argument("--threads", type=int, default=0, help="batch: host threads for init (0: min(16, cores))")
+    ap.add_argument("--tune", action="store_true", help="batch: autotune the tile per grid size class first")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES through the HIP runtime "
                          "(calibrates the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -341,6 +355,11 @@ def main() -> int:
         except Exception:
             traffic = None
 
+    # whole step: both species' push bytes + current reset (12 B/cell), two
+    # filter passes (24 B/cell each) and the fused Yee update (60 B/cell)
+    step_bytes = 2 * bytes_per_launch + (12.0 + 2 * 24.0 + 60.0) * cells
+    step_achieved = step_bytes / (ms_step * 1e-3) / 1e9
+
     cpu = None
     if not args.no_cpu:
         nproc = args.cpu_procs or min(16, os.cpu_count() or 1)
@@ -366,9 +385,13 @@ def main() -> int:
                    "nx": nx, "ppc": ppc, "particles": npart, "steps_total": args.warmup + args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "push_kernel (push + deposit, one launch per species per step)",
+                     "kernel": "push_tile_kernel + push_global_kernel (push + deposit, one species)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": 1e3 * avg_push},
+        "step_roofline": {"bound": "hbm", "achieved": step_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_achieved / HBM_PEAK_GBS, "bytes_per_step": step_bytes},
         "cpu_baseline": cpu,
+        "vs_cpu": (value / cpu["value"]) if cpu else None,
+        "vs_cpu_single_core": (value / cpu["single_core_value"]) if cpu else None,
         "kernels": kt,
         "overflow_pushes": overflow,
     }

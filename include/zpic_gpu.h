@@ -33,6 +33,9 @@ void zpic_set_device(int device);
  * must divide the grid; a side that no preferred tile width divides uses
  * the largest divisor <= 64 that leaves room for the margin. */
 void zpic_set_tiling(int tx, int ty, int halo);
+/* The same for one simulation only (its species and current), before its
+ * first step; 0 entries keep the process-wide setting. */
+void zpic_sim_set_tiling(t_simulation *sim, int tx, int ty, int halo);
 
 /* Drop the deviate the normal generator keeps for its next call.
  * set_rand_seed keeps it, as the reference does (em2d/random.c:25-29,
@@ -40,8 +43,9 @@ void zpic_set_tiling(int tx, int ty, int halo);
  * identical generator states. */
 void zpic_rand_clear(void);
 
-/* Occupancy variant of the push kernel: 0 (default) keeps every value in
- * registers, 8 requests 8 waves per SIMD with a few register spills. */
+/* Variant of the push kernel: 0 (default) keeps every value in registers,
+ * 8 requests 8 waves per SIMD with a few register spills, 2 pushes two
+ * particles per thread and iteration. */
 void zpic_set_push_waves(int waves);
 
 /* Block until all queued GPU work is done. */

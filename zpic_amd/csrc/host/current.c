@@ -61,7 +61,7 @@ void zpic_current_ensure(t_current *c, const zk_tiles *t)
     if (!d->ready) {
         zk_tiles tt;
         if (t) tt = *t;
-        else zpic_tiles_for(c->nx[0], c->nx[1], &tt);
+        else zpic_tiles_pref(c->nx[0], c->nx[1], d->pref[0], d->pref[1], d->pref[2], &tt);
         d->t = tt;
         const size_t nodes = zpic_grid_nodes(c->nx);
         d->J[0] = zpic_dmalloc(sizeof(float3) * nodes);

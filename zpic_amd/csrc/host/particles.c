@@ -469,7 +469,7 @@ void zpic_species_ensure(t_species *s)
     if (!d) zpic_fatal("species '%s' was not created with spec_new", s->name);
     if (d->host_dirty) validate_host_particles(s);
     if (!d->ready) {
-        zpic_tiles_for(s->nx[0], s->nx[1], &d->t);
+        zpic_tiles_pref(s->nx[0], s->nx[1], d->pref[0], d->pref[1], d->pref[2], &d->t);
         int want = s->np > 0 ? s->np : 1;
         if (s->moving_window) {
             /* a full box of particles plus a few columns of injections */

@@ -132,13 +132,16 @@ static int pick_tile(int n, int want, int halo)
     return best;
 }
 
-void zpic_tiles_for(int nx, int ny, zk_tiles *t)
+void zpic_tiles_for(int nx, int ny, zk_tiles *t) { zpic_tiles_pref(nx, ny, 0, 0, 0, t); }
+
+void zpic_tiles_pref(int nx, int ny, int want_x, int want_y, int want_h, zk_tiles *t)
 {
-    int h = g_halo;
+    const int wx = want_x > 0 ? want_x : g_tx, wy = want_y > 0 ? want_y : g_ty;
+    int h = want_h > 0 ? want_h : g_halo;
     int tx = -1, ty = -1;
     for (; h >= 0; h--) {
-        tx = pick_tile(nx, g_tx, h);
-        ty = pick_tile(ny, g_ty, h);
+        tx = pick_tile(nx, wx, h);
+        ty = pick_tile(ny, wy, h);
         if (tx >= h + 3 && ty >= h + 3) break;
     }
     if (tx < 3 || ty < 3 || h < 0)

@@ -112,3 +112,23 @@ void sim_delete(t_simulation *sim)
     current_delete(&sim->current);
     emf_delete(&sim->emf);
 }
+
+void zpic_sim_set_tiling(t_simulation *sim, int tx, int ty, int halo)
+{
+    /* takes effect for device state not created yet (before the first step) */
+    for (int k = 0; k < sim->n_species; k++) {
+        struct zpic_dev_species *d = sim->species[k].dev;
+        if (!d) continue;
+        if (d->ready) zpic_fatal("zpic_sim_set_tiling: species '%s' is already on the device", sim->species[k].name);
+        d->pref[0] = tx;
+        d->pref[1] = ty;
+        d->pref[2] = halo;
+    }
+    struct zpic_dev_current *c = sim->current.dev;
+    if (c) {
+        if (c->ready) zpic_fatal("zpic_sim_set_tiling: the current density is already on the device");
+        c->pref[0] = tx;
+        c->pref[1] = ty;
+        c->pref[2] = halo;
+    }
+}

@@ -46,6 +46,7 @@ struct zpic_dev_species {
     int host_ids_cap;
     struct zpic_dev_species *next_live;
     t_species *owner;
+    int pref[3];          /* preferred tile (tx, ty, halo), 0: process-wide setting */
 };
 
 struct zpic_dev_current {
@@ -57,6 +58,7 @@ struct zpic_dev_current {
     float *region;        /* per-tile current windows */
     float *Jov;           /* raw current of out-of-window pushes */
     int fresh;            /* no deposit since current_zero */
+    int pref[3];          /* preferred tile (tx, ty, halo), 0: process-wide setting */
 };
 
 struct zpic_dev_emf {

@@ -84,12 +84,64 @@ class Member:
             self.sim = None
 
 
-def build_members(seeds, threads: int = 16) -> list[Member]:
-    """Members for the given seeds, initialised on `threads` host threads."""
+def build_members(seeds, threads: int = 16, tiles: dict | None = None) -> list[Member]:
+    """Members for the given seeds, initialised on `threads` host threads;
+    `tiles` maps a size class (nx, ny) to the tile (tx, ty, halo) to use."""
     lib = _lib.load()
     seeds = list(seeds)
+
+    def make(s):
+        m = Member(lib, member_params(s))
+        if tiles:
+            t = tiles.get(size_class(m.params))
+            if t:
+                lib.zpic_sim_set_tiling(C.addressof(m.sim), *t)
+        return m
+
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
-        return list(ex.map(lambda s: Member(lib, member_params(s)), seeds))
+        return list(ex.map(make, seeds))
+
+
+# ---- tile autotuning per size class ----
+
+# A batch launches its push with the largest LDS window of its members, so
+# the candidates keep the window within the default's (16 + 2*3 + 3 = 25).
+TILE_CANDIDATES = ((16, 16, 3), (16, 16, 2), (16, 8, 3), (8, 16, 3), (8, 8, 3), (8, 8, 2))
+
+
+def size_class(params: dict) -> tuple:
+    return (params["nx"], params["ny"])
+
+
+def tune_tiles(classes, members: int = 8, steps: int = 10, candidates=TILE_CANDIDATES) -> dict:
+    """The fastest tile of `candidates` for each size class (nx, ny),
+    timed on a batch of `members` mid-range members of that class (p = 5,
+    uz = 0.35, uth = 0.05; generator seeds outside the scan's) for `steps`
+    steps after one warm-up step."""
+    import time
+
+    lib = _lib.load()
+    best = {}
+    for nx, ny in sorted(set(classes)):
+        times = {}
+        for cand in candidates:
+            ms = []
+            for k in range(members):
+                m = Member(lib, {"seed": 100000 + k, "nx": nx, "ny": ny, "ppc": 5, "uz": 0.35, "uth": 0.05})
+                lib.zpic_sim_set_tiling(C.addressof(m.sim), *cand)
+                ms.append(m)
+            b = Batch(ms)
+            b.run(1)
+            lib.zpic_synchronize()
+            t0 = time.perf_counter()
+            b.run(steps)
+            lib.zpic_synchronize()
+            times[cand] = time.perf_counter() - t0
+            b.close()
+            for m in ms:
+                m.close()
+        best[(nx, ny)] = min(times, key=times.get)
+    return best
 
 
 class Batch:
