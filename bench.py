@@ -249,6 +249,53 @@ def adjoint_workload(args) -> dict:
     }
 
 
+def lwfa_workload(args) -> dict:
+    """Laser wakefield with a moving window: 4000 x 256 cells, dx = 0.05,
+    dy = 0.2, dt = 0.045; electrons (q/m = -1, 4 x 2 ppc, cold) from
+    x = 20 on; Gaussian laser a0 = 2, omega0 = 10, W0 = 4, FWHM 2, start 17,
+    focus 20, axis 25.6, polarization 0; 4-pass compensated filter along
+    x; 2000 steps (10 untimed)."""
+    import ctypes as C
+
+    from zpic_amd import _lib
+    from zpic_amd import em2d
+
+    lib = _lib.load()
+    lib.zpic_set_push_waves(args.waves)
+    nx, ny, dx, dy, dt = 4000, 256, 0.05, 0.2, 0.045
+    log("lwfa: building 4000x256 with moving window")
+    e = em2d.Species("electrons", -1.0, [4, 2], density=em2d.Density(type="step", start=20.0))
+    sim = em2d.Simulation([nx, ny], [nx * dx, ny * dy], dt, species=[e], mov_window=True,
+                          smooth=em2d.Smooth(xtype="compensated", xlevel=4))
+    sim.add_laser(em2d.Laser(type="gaussian", start=17.0, fwhm=2.0, a0=2.0, omega0=10.0, W0=4.0, focus=20.0,
+                             axis=25.6, polarization=0.0))
+    npart = int(e._s.np)
+    warm, steps = 10, 1990
+    for _ in range(warm):
+        sim.iter()
+    lib.zpic_synchronize()
+    n0 = lib.spec_npush()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        sim.iter()
+        if (k + 1) % 500 == 0:
+            log(f"lwfa: queued {k + 1} steps")
+    lib.zpic_synchronize()
+    el = time.perf_counter() - t0
+    pushes = lib.spec_npush() - n0
+    sim.close()
+    return {
+        "metric": "particle-steps/s", "value": pushes / el, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": warm, "ms_per_step": 1e3 * el / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: STEP plasma and Gaussian laser from the em2d initialisers",
+        "config": {"workload": "em2d LWFA 4000x256, dx=0.05 dy=0.2 dt=0.045, moving window, 4x2 ppc from x=20, "
+                               "Gaussian laser a0=2 omega0=10 W0=4 fwhm=2, compensated 4-pass x filter",
+                   "particles_at_start": npart},
+        "pushes": int(pushes),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,7 +311,7 @@ def main() -> int:
     ap.add_argument("--cpu-nx", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
-    ap.add_argument("--workload", choices=["weibel", "batch", "adjoint"], default="weibel",
+    ap.add_argument("--workload", choices=["weibel", "batch", "adjoint", "lwfa"], default="weibel",
                     help="weibel: the 2048^2 Weibel step (default); batch: the batched parameter scan; "
                          "adjoint: backward/forward time of the differentiable 256^2 Weibel run")
     ap.add_argument("--members", type=int, default=512, help="batch: members of the scan")
@@ -291,6 +338,9 @@ def main() -> int:
         return 0
     if args.workload == "adjoint":
         print(json.dumps(adjoint_workload(args)), flush=True)
+        return 0
+    if args.workload == "lwfa":
+        print(json.dumps(lwfa_workload(args)), flush=True)
         return 0
     import ctypes as C
 
