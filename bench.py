@@ -306,7 +306,8 @@ def main() -> int:
     ap.add_argument("--n-sort", type=int, default=20)
     ap.add_argument("--tile", type=int, default=0, help="tile width in cells (0: library default)")
     ap.add_argument("--halo", type=int, default=0, help="tile drift margin in cells (0: library default)")
-    ap.add_argument("--waves", type=int, default=0, help="push occupancy variant: 0 (default) or 8")
+    ap.add_argument("--waves", type=int, default=0, help="push variant: 0 (default), 2 or 8")
+    ap.add_argument("--fixed", type=int, default=-1, help="fixed-point current deposit: 1 on, 0 off (-1: library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-nx", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=10)
@@ -351,6 +352,8 @@ def main() -> int:
     if args.tile or args.halo:
         lib.zpic_set_tiling(args.tile, args.tile, args.halo)
     lib.zpic_set_push_waves(args.waves)
+    if args.fixed >= 0:
+        lib.zpic_set_fixed_deposit(args.fixed)
     nx = args.nx
     ppc = args.ppc
     dx, dt, uz, uth = 0.1, 0.07, 0.6, 0.1

@@ -30,6 +30,7 @@ def load() -> C.CDLL:
         "zpic_set_tiling": (None, [C.c_int, C.c_int, C.c_int]),
         "zpic_sim_set_tiling": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "zpic_set_push_waves": (None, [C.c_int]),
+        "zpic_set_fixed_deposit": (None, [C.c_int]),
         "zpic_synchronize": (None, []),
         "zpic_rand_clear": (None, []),
         "spec_sync_host": (None, [C.c_void_p]),

@@ -638,6 +638,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     a.ovf_list = d->ovf_list;
     a.global_blocks = tb;
     a.waves = zpic_push_waves();
+    a.fixed = zpic_fixed_deposit();
     zpic_ktimer_begin(ZPIC_T_PUSH);
     ZCHECK(zk_push(&a, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_PUSH);

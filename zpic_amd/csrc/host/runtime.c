@@ -27,6 +27,18 @@ static int g_push_waves = 0;
 void zpic_set_push_waves(int waves) { g_push_waves = (waves == 8 || waves == 2) ? waves : 0; }
 int zpic_push_waves(void) { return g_push_waves; }
 
+/* -1: not chosen yet (environment variable ZPIC_FIXED_DEPOSIT, default off) */
+static int g_fixed_deposit = -1;
+void zpic_set_fixed_deposit(int on) { g_fixed_deposit = on ? 1 : 0; }
+int zpic_fixed_deposit(void)
+{
+    if (g_fixed_deposit < 0) {
+        const char *e = getenv("ZPIC_FIXED_DEPOSIT");
+        g_fixed_deposit = (e && e[0] == '1') ? 1 : 0;
+    }
+    return g_fixed_deposit;
+}
+
 void zpic_check(int status, const char *what, const char *file, int line)
 {
     if (status != 0) {

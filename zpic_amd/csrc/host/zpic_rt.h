@@ -37,6 +37,7 @@ void zpic_tiles_for(int nx, int ny, zk_tiles *t);
 /* the same with a preferred tile (0 entries: the process-wide setting) */
 void zpic_tiles_pref(int nx, int ny, int want_x, int want_y, int want_h, zk_tiles *t);
 int zpic_push_waves(void);
+int zpic_fixed_deposit(void);
 size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */
 
 /* GPU timers: spec_time() / emf_time(), and per-kernel-class timers that

@@ -31,10 +31,10 @@ using namespace zk;
 
 namespace {
 
-template <int MIN_WAVES, int PAIR>
+template <int MIN_WAVES, int PAIR, bool FX>
 __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args a)
 {
-    push_tile_body<PAIR>(a, zk_xcd_item(blockIdx.x, a.t.ntiles));
+    push_tile_body<PAIR, FX>(a, zk_xcd_item(blockIdx.x, a.t.ntiles));
 }
 
 __global__ void __launch_bounds__(256) push_global_kernel(zk_push_args a)
@@ -59,25 +59,30 @@ __global__ void fill_kernel(float *p, size_t n, float v)
 extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
 {
     const int n = a->t.WX * a->t.WY;
-    const size_t lds = (size_t)9 * n * sizeof(float);
+    /* E, B window (6 n floats) and the current window: 3 n floats, or 3 n
+     * 64-bit integers with the fixed-point deposit */
+    const size_t lds = (size_t)6 * n * sizeof(float) + (size_t)3 * n * (a->fixed ? 8 : 4);
     /* variants: 0 lets the compiler keep every value in registers (about
      * 6 waves per SIMD); 8 asks for 8 waves per SIMD at the price of a few
      * register spills; 2 pushes two particles per thread and iteration */
-    const void *fn = a->waves == 8   ? (const void *)push_tile_kernel<8, 1>
-                     : a->waves == 2 ? (const void *)push_tile_kernel<1, 2>
-                                     : (const void *)push_tile_kernel<1, 1>;
+    const void *fn = a->fixed         ? (const void *)push_tile_kernel<1, 1, true>
+                     : a->waves == 8 ? (const void *)push_tile_kernel<8, 1, false>
+                     : a->waves == 2 ? (const void *)push_tile_kernel<1, 2, false>
+                                     : (const void *)push_tile_kernel<1, 1, false>;
     if (lds > 65536) {
         /* windows of large tiles need more than the default 64 KiB of LDS */
         const int e = (int)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e) return e;
     }
     if (a->t.ntiles > 0) {
-        if (a->waves == 8)
-            hipLaunchKernelGGL((push_tile_kernel<8, 1>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+        if (a->fixed)
+            hipLaunchKernelGGL((push_tile_kernel<1, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+        else if (a->waves == 8)
+            hipLaunchKernelGGL((push_tile_kernel<8, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else if (a->waves == 2)
-            hipLaunchKernelGGL((push_tile_kernel<1, 2>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<1, 2, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else
-            hipLaunchKernelGGL((push_tile_kernel<1, 1>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<1, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
     }
     if (a->global_blocks > 0) hipLaunchKernelGGL(push_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
     ZK_LAUNCH_CHECK();

@@ -6,6 +6,8 @@
 #ifndef ZK_PUSH_CORE_H
 #define ZK_PUSH_CORE_H
 
+#include <type_traits>
+
 #include "common.h"
 
 
@@ -44,6 +46,32 @@ struct LdsCurrent {
     float *j;
     int w, n;
     __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], v); }
+};
+
+/* Fixed-point current sink: contributions are scaled by 2^40, rounded to
+ * 64-bit integers and summed with integer LDS atomics, so the window sum is
+ * exact and independent of the order of the atomics. */
+constexpr float ZK_FX_SCALE = 1099511627776.0f;        /* 2^40 */
+constexpr float ZK_FX_UNSCALE = 9.094947017729282e-13f; /* 2^-40 */
+
+/* v * 2^40 as a 64-bit integer, from the float's fields: exact for
+ * |v| >= 2^-16, truncated toward zero below (an error under 2^-40);
+ * |v| must stay below 2^15 (current contributions are O(q)). */
+__device__ inline unsigned long long zk_fx(float v)
+{
+    const unsigned u = __float_as_uint(v);
+    const int e = (int)((u >> 23) & 0xffu);
+    const unsigned long long m = (unsigned long long)((u & 0x7fffffu) | 0x800000u);
+    const int sh = e - 110; /* m * 2^(e - 150) * 2^40 */
+    unsigned long long r = sh >= 0 ? (m << (sh < 39 ? sh : 39)) : (sh > -24 ? (m >> -sh) : 0ull);
+    r = e ? r : 0ull;
+    return (u >> 31) ? (0ull - r) : r;
+}
+
+struct LdsCurrentFx {
+    unsigned long long *j;
+    int w, n;
+    __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], zk_fx(v)); }
 };
 
 struct GlobalCurrent {
@@ -310,8 +338,9 @@ __device__ inline void list_out(const zk_push_args &a, int i0, int i, bool out_h
 
 /* Tile pass.  PAIR = 2 pushes two particles per thread and iteration: the
  * gathers and Boris pushes of both are independent and interleave, which
- * hides LDS and arithmetic latency at the price of registers. */
-template <int PAIR = 1>
+ * hides LDS and arithmetic latency at the price of registers.  FX selects
+ * the fixed-point window (LdsCurrentFx) instead of float LDS atomics. */
+template <int PAIR = 1, bool FX = false>
 __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
 {
     extern __shared__ float sm[];
@@ -329,6 +358,7 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H; /* cell index of window node 0 */
     float *fl = sm;
     float *jl = sm + 6 * n;
+    unsigned long long *jx = (unsigned long long *)(sm + 6 * n); /* FX: 3 n integers */
     /* stage the window of E and B; nodes outside the guard-inclusive
      * buffer are never read by an in-window particle */
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
@@ -341,13 +371,24 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
         fl[3 * n + m] = in ? a.B[g + 0] : 0.0f;
         fl[4 * n + m] = in ? a.B[g + 1] : 0.0f;
         fl[5 * n + m] = in ? a.B[g + 2] : 0.0f;
-        jl[0 * n + m] = 0.0f;
-        jl[1 * n + m] = 0.0f;
-        jl[2 * n + m] = 0.0f;
+        if (FX) {
+            jx[0 * n + m] = 0ull;
+            jx[1 * n + m] = 0ull;
+            jx[2 * n + m] = 0ull;
+        } else {
+            jl[0 * n + m] = 0.0f;
+            jl[1 * n + m] = 0.0f;
+            jl[2 * n + m] = 0.0f;
+        }
     }
     __syncthreads();
     const LdsFields lf = {fl, t.WX, n};
-    const LdsCurrent lj = {jl, t.WX, n};
+    typedef typename std::conditional<FX, LdsCurrentFx, LdsCurrent>::type Sink;
+    Sink lj;
+    lj.w = t.WX;
+    lj.n = n;
+    if constexpr (FX) lj.j = jx;
+    else lj.j = jl;
     const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
     if (PAIR == 1) {
         for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -433,10 +474,9 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     }
     /* store the current window of this tile */
     float *dst = a.region + (size_t)tile * 3 * n;
-    if (a.fresh) {
-        for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] = jl[m];
-    } else {
-        for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) dst[m] += jl[m];
+    for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) {
+        const float v = FX ? (float)(long long)jx[m] * ZK_FX_UNSCALE : jl[m];
+        dst[m] = a.fresh ? v : dst[m] + v;
     }
     /* per-tile energy partial (summed in fixed order by zk_spec_finalize) */
     const double s = zk_block_sum(kin, red);

@@ -71,7 +71,8 @@ typedef struct zk_push_args {
     int *ovf_list;              /* slots of out-of-window particles: tile t
                                    lists its own at tile_off[t] onwards */
     int global_blocks;          /* blocks of the global (tail) pass      */
-    int waves;                  /* tile-pass occupancy variant: 0 or 8   */
+    int waves;                  /* tile-pass variant: 0, 2 or 8           */
+    int fixed;                  /* 1: fixed-point (int64) current window  */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
 
