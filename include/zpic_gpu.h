@@ -48,10 +48,12 @@ void zpic_rand_clear(void);
  * particles per thread and iteration. */
 void zpic_set_push_waves(int waves);
 
-/* Current deposit of the push: 0 (default) sums the tile windows with
- * float LDS atomics; 1 sums them exactly in 64-bit fixed point (2^-40
- * units) with integer LDS atomics, independent of the order of the
- * atomics, and rounds each window node once. */
+/* Current deposit of the push: 1 (default; environment variable
+ * ZPIC_FIXED_DEPOSIT=0 turns it off) sums each tile window exactly in
+ * 64-bit fixed point (units of 2^-40 Q, Q the power of two at or above the
+ * particle charge) with integer LDS atomics, independent of the order of
+ * the atomics, and rounds each window node once; 0 sums the windows with
+ * float LDS atomics. */
 void zpic_set_fixed_deposit(int on);
 
 /* Block until all queued GPU work is done. */

@@ -134,19 +134,19 @@ def test_adjoint_matches_central_differences_of_gpu_runs():
     """Central differences (h = 1e-3) of the GPU forward loss against the
     adjoint gradient: for uz, for uth, and for the seed along the unit
     vector of its adjoint gradient (the directional derivative must equal
-    the gradient's norm).
+    the gradient's norm), on the full 256 x 256 configuration.
 
-    The loss is only piecewise smooth in the parameters (its derivative
-    jumps where a particle crosses an interpolation node or cell edge), so
-    differences over a finite h average many such jumps: in exact FP64
-    arithmetic on the 32 x 32 subcase, central differences with h = 1e-3
-    differ from the autograd derivative by 5e-4 (uz) and 1.3e-2 (uth).
-    The FP32 loss also carries rounding noise of ~1e-7 relative that
-    changes between runs (deposit atomics in LDS), which is why a single
-    seed cell, whose derivative is ~100 times smaller than the norm, is not
-    used.  The check runs on the full 256 x 256 configuration, where the
-    loss sums many modes and these effects are smallest.  Bars: 1e-3 for uz
-    and the seed direction, 2e-2 for uth."""
+    The loss is only piecewise smooth in the parameters: its derivative
+    jumps wherever a particle crosses an interpolation node or a cell edge,
+    and with 10^4-10^6 particles over 200 steps such kinks lie closer
+    together than any usable h, so a central difference is the slope of a
+    chord across many of them, not the derivative.  Measured in exact FP64
+    arithmetic (autograd vs central differences through the FP64
+    restatement, 32 x 32 subcase): uz 5e-4 at h = 1e-3; uth 1.3e-2; the
+    seed direction 3.5e-3 at h = 1e-3 and 8e-3 at h = 1e-4 (no convergence
+    as h shrinks).  Bars: 1e-3 for uz, 2e-2 for uth and 1e-2 for the seed
+    direction; the exact checks of the reverse step are the FP64 ones
+    above."""
     cfg = WeibelConfig()  # 256 x 256, 2 x 4x4 ppc, 200 steps, ckpt 20, loss 150-200, 64 x 64 seed
     seed = 1e-3 * np.random.default_rng(5).standard_normal((64, 64))
     uz, uth, h = 0.6, 0.1, 1e-3
@@ -175,4 +175,4 @@ def test_adjoint_matches_central_differences_of_gpu_runs():
           flush=True)
     assert abs(g_uz - fd_uz) <= 1e-3 * abs(fd_uz)
     assert abs(g_uth - fd_uth) <= 2e-2 * abs(fd_uth)
-    assert abs(g_s - fd_s) <= 1e-3 * abs(fd_s)
+    assert abs(g_s - fd_s) <= 1e-2 * abs(fd_s)

@@ -48,7 +48,7 @@ struct zpic_batch {
     int *gblocks, *sblocks;  /* per species: blocks of the out-of-window pass and of the sort */
     int *ntiles;             /* per simulation, at creation */
     size_t lds;
-    int waves;
+    int waves, fixed;
     void *graph[2];     /* 0: plain step, 1: step with the tile sort */
 };
 
@@ -114,6 +114,8 @@ static void fill_spec(zpic_batch *b, int k)
     a->ovf_list = d->ovf_list;
     a->global_blocks = b->gblocks[k];
     a->waves = b->waves;
+    a->fixed = b->fixed;
+    zpic_fixed_point_scale(s->q, &a->fx_base, &a->fx_unscale);
     a->tem = 0.5 * s->dt / s->m_q;
     a->dt_dx = s->dt / s->dx[0];
     a->dt_dy = s->dt / s->dx[1];
@@ -270,7 +272,8 @@ static void enqueue_step(zpic_batch *b, int with_sort)
     /* spec_advance of the p-th species of every simulation; the species
      * of one simulation deposit into one set of windows, in order */
     for (int p = 0; p < b->max_spec; p++)
-        ZCHECK(zk_batch_push(b->dspecs, b->dsims, ITEMS(b->tile[p]), ITEMS(b->glob[p]), b->lds, b->waves, s));
+        ZCHECK(zk_batch_push(b->dspecs, b->dsims, ITEMS(b->tile[p]), ITEMS(b->glob[p]), b->lds, b->waves, b->fixed,
+                             s));
     ZCHECK(zk_batch_finalize(b->dspecs, b->nspecs, s));
     if (with_sort) ZCHECK(zk_batch_sort(b->dspecs, b->nspecs, ITEMS(b->sort), s));
     /* current_update, emf_advance */
@@ -319,6 +322,7 @@ zpic_batch *zpic_batch_new(t_simulation *const *sims, int n)
     b->dsims = zpic_dmalloc(sizeof(zk_bsim) * (size_t)n);
     b->dspecs = zpic_dmalloc(sizeof(zk_bspec) * (size_t)ns);
     b->waves = zpic_push_waves();
+    b->fixed = zpic_fixed_deposit();
     /* block counts of the particle passes are fixed with the item tables */
     b->gblocks = malloc(sizeof(int) * (size_t)ns);
     b->sblocks = malloc(sizeof(int) * (size_t)ns);
@@ -334,7 +338,8 @@ zpic_batch *zpic_batch_new(t_simulation *const *sims, int n)
     b->lds = 0;
     for (int k = 0; k < b->nspecs; k++) {
         const zk_tiles *t = &b->hspecs[k].a.t;
-        const size_t l = (size_t)9 * t->WX * t->WY * sizeof(float);
+        /* E, B window and the current window (64-bit with the fixed-point deposit) */
+        const size_t l = (size_t)t->WX * t->WY * (6 * sizeof(float) + 3 * (b->fixed ? 8 : 4));
         if (l > b->lds) b->lds = l;
     }
     for (int i = 0; i < n; i++)

@@ -5,6 +5,7 @@
  * (process-global timers and RNG, em2d/particles.c:26-27, em2d/emf.c:27,
  * em2d/random.c:16-17), so one process-wide stream carries all work.
  */
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -27,16 +28,29 @@ static int g_push_waves = 0;
 void zpic_set_push_waves(int waves) { g_push_waves = (waves == 8 || waves == 2) ? waves : 0; }
 int zpic_push_waves(void) { return g_push_waves; }
 
-/* -1: not chosen yet (environment variable ZPIC_FIXED_DEPOSIT, default off) */
+/* -1: not chosen yet (environment variable ZPIC_FIXED_DEPOSIT, default on) */
 static int g_fixed_deposit = -1;
 void zpic_set_fixed_deposit(int on) { g_fixed_deposit = on ? 1 : 0; }
 int zpic_fixed_deposit(void)
 {
     if (g_fixed_deposit < 0) {
         const char *e = getenv("ZPIC_FIXED_DEPOSIT");
-        g_fixed_deposit = (e && e[0] == '1') ? 1 : 0;
+        g_fixed_deposit = (e && e[0] == '0') ? 0 : 1;
     }
     return g_fixed_deposit;
+}
+
+void zpic_fixed_point_scale(float q, int *base, float *unscale)
+{
+    /* Q = 2^k, the power of two at or above |q| */
+    int k = 0;
+    const float aq = fabsf(q);
+    if (aq > 0.0f) {
+        frexpf(aq, &k); /* aq = f 2^k, f in [0.5, 1) */
+        if (ldexpf(1.0f, k - 1) == aq) k -= 1;
+    }
+    *base = 110 + k;
+    *unscale = ldexpf(1.0f, k - 40);
 }
 
 void zpic_check(int status, const char *what, const char *file, int line)

@@ -48,13 +48,13 @@ __global__ void fill_jov_kernel(const zk_bsim *sims, const int *items)
     if (i < n) M.Jov[i] = 0.0f;
 }
 
-template <int MIN_WAVES>
+template <int MIN_WAVES, bool FX>
 __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_batch(const zk_bspec *specs, const zk_bsim *sims,
                                                                   const int *items)
 {
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     const zk_push_args a = spec_args(S, sims[S.sim]);
-    push_tile_body(a, items[2 * blockIdx.x + 1]);
+    push_tile_body<1, FX>(a, items[2 * blockIdx.x + 1]);
 }
 
 __global__ void __launch_bounds__(256) push_global_batch(const zk_bspec *specs, const zk_bsim *sims,
@@ -166,13 +166,15 @@ extern "C" int zk_batch_fill_jov(const zk_bsim *sims, const int *items, int n, z
 }
 
 extern "C" int zk_batch_push(const zk_bspec *specs, const zk_bsim *sims, const int *tile_items, int ntile,
-                             const int *global_items, int nglobal, size_t lds, int waves, zk_stream_t s)
+                             const int *global_items, int nglobal, size_t lds, int waves, int fixed, zk_stream_t s)
 {
     if (ntile > 0) {
-        if (waves == 8)
-            hipLaunchKernelGGL(push_tile_batch<8>, dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
+        if (fixed)
+            hipLaunchKernelGGL((push_tile_batch<1, true>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
+        else if (waves == 8)
+            hipLaunchKernelGGL((push_tile_batch<8, false>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
         else
-            hipLaunchKernelGGL(push_tile_batch<1>, dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
+            hipLaunchKernelGGL((push_tile_batch<1, false>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
     }
     LAUNCH_ITEMS(push_global_batch, nglobal, specs, sims, global_items);
     ZK_LAUNCH_CHECK();
@@ -181,12 +183,13 @@ extern "C" int zk_batch_push(const zk_bspec *specs, const zk_bsim *sims, const i
 extern "C" int zk_batch_prepare_push(size_t lds)
 {
     if (lds <= 65536) return 0;
-    int e = (int)hipFuncSetAttribute((const void *)push_tile_batch<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
-    if (!e)
-        e = (int)hipFuncSetAttribute((const void *)push_tile_batch<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
-    return e;
+    const void *fns[3] = {(const void *)push_tile_batch<1, false>, (const void *)push_tile_batch<8, false>,
+                          (const void *)push_tile_batch<1, true>};
+    for (int k = 0; k < 3; k++) {
+        const int e = (int)hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e) return e;
+    }
+    return 0;
 }
 
 extern "C" int zk_batch_finalize(const zk_bspec *specs, int nspecs, zk_stream_t s)

@@ -48,21 +48,19 @@ struct LdsCurrent {
     __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], v); }
 };
 
-/* Fixed-point current sink: contributions are scaled by 2^40, rounded to
- * 64-bit integers and summed with integer LDS atomics, so the window sum is
- * exact and independent of the order of the atomics. */
-constexpr float ZK_FX_SCALE = 1099511627776.0f;        /* 2^40 */
-constexpr float ZK_FX_UNSCALE = 9.094947017729282e-13f; /* 2^-40 */
-
-/* v * 2^40 as a 64-bit integer, from the float's fields: exact for
- * |v| >= 2^-16, truncated toward zero below (an error under 2^-40);
- * |v| must stay below 2^15 (current contributions are O(q)). */
-__device__ inline unsigned long long zk_fx(float v)
+/* Fixed-point current sink: contributions are divided by Q (the power of
+ * two at or above the particle charge |q|; every contribution is O(|q|)),
+ * scaled by 2^40, rounded toward zero to 64-bit integers and summed with
+ * integer LDS atomics, so a window's sum is exact and independent of the
+ * order of the atomics.  zk_fx(v, base) = v / Q * 2^40 with
+ * base = 110 + log2(Q), computed from the float's fields: exact when
+ * |v / Q| >= 2^-16, within 2^-40 below; |v / Q| must stay below 2^16. */
+__device__ inline unsigned long long zk_fx(float v, int base)
 {
     const unsigned u = __float_as_uint(v);
     const int e = (int)((u >> 23) & 0xffu);
     const unsigned long long m = (unsigned long long)((u & 0x7fffffu) | 0x800000u);
-    const int sh = e - 110; /* m * 2^(e - 150) * 2^40 */
+    const int sh = e - base;
     unsigned long long r = sh >= 0 ? (m << (sh < 39 ? sh : 39)) : (sh > -24 ? (m >> -sh) : 0ull);
     r = e ? r : 0ull;
     return (u >> 31) ? (0ull - r) : r;
@@ -70,8 +68,8 @@ __device__ inline unsigned long long zk_fx(float v)
 
 struct LdsCurrentFx {
     unsigned long long *j;
-    int w, n;
-    __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], zk_fx(v)); }
+    int w, n, base;
+    __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], zk_fx(v, base)); }
 };
 
 struct GlobalCurrent {
@@ -387,8 +385,12 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     Sink lj;
     lj.w = t.WX;
     lj.n = n;
-    if constexpr (FX) lj.j = jx;
-    else lj.j = jl;
+    if constexpr (FX) {
+        lj.j = jx;
+        lj.base = a.fx_base;
+    } else {
+        lj.j = jl;
+    }
     const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
     if (PAIR == 1) {
         for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -475,7 +477,7 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     /* store the current window of this tile */
     float *dst = a.region + (size_t)tile * 3 * n;
     for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) {
-        const float v = FX ? (float)(long long)jx[m] * ZK_FX_UNSCALE : jl[m];
+        const float v = FX ? (float)(long long)jx[m] * a.fx_unscale : jl[m];
         dst[m] = a.fresh ? v : dst[m] + v;
     }
     /* per-tile energy partial (summed in fixed order by zk_spec_finalize) */

@@ -73,6 +73,8 @@ typedef struct zk_push_args {
     int global_blocks;          /* blocks of the global (tail) pass      */
     int waves;                  /* tile-pass variant: 0, 2 or 8           */
     int fixed;                  /* 1: fixed-point (int64) current window  */
+    int fx_base;                /* fixed point: 110 + log2(Q), Q = 2^ceil(log2|q|) */
+    float fx_unscale;           /* fixed point: 2^(fx_base - 150)          */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
 
@@ -162,7 +164,7 @@ typedef struct zk_bspec {
 int zk_batch_fill_jov(const zk_bsim *sims, const int *items, int n, zk_stream_t s);
 int zk_batch_prepare_push(size_t lds);
 int zk_batch_push(const zk_bspec *specs, const zk_bsim *sims, const int *tile_items, int ntile,
-                  const int *global_items, int nglobal, size_t lds, int waves, zk_stream_t s);
+                  const int *global_items, int nglobal, size_t lds, int waves, int fixed, zk_stream_t s);
 int zk_batch_finalize(const zk_bspec *specs, int nspecs, zk_stream_t s);
 int zk_batch_current(const zk_bsim *sims, const int *node_items, int n, int max_passes, zk_stream_t s);
 int zk_batch_fields(const zk_bsim *sims, const int *yee_items, int nyee, const int *guard_items, int nguard,
