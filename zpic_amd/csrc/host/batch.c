@@ -115,7 +115,7 @@ static void fill_spec(zpic_batch *b, int k)
     a->global_blocks = b->gblocks[k];
     a->waves = b->waves;
     a->fixed = b->fixed;
-    zpic_fixed_point_scale(s->q, &a->fx_base, &a->fx_unscale);
+    zpic_fixed_point_scale(s->q, &a->fx_scale, &a->fx_unscale);
     a->tem = 0.5 * s->dt / s->m_q;
     a->dt_dx = s->dt / s->dx[0];
     a->dt_dy = s->dt / s->dx[1];

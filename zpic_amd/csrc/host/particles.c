@@ -639,7 +639,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     a.global_blocks = tb;
     a.waves = zpic_push_waves();
     a.fixed = zpic_fixed_deposit();
-    zpic_fixed_point_scale(s->q, &a.fx_base, &a.fx_unscale);
+    zpic_fixed_point_scale(s->q, &a.fx_scale, &a.fx_unscale);
     zpic_ktimer_begin(ZPIC_T_PUSH);
     ZCHECK(zk_push(&a, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_PUSH);

@@ -40,7 +40,7 @@ int zpic_fixed_deposit(void)
     return g_fixed_deposit;
 }
 
-void zpic_fixed_point_scale(float q, int *base, float *unscale)
+void zpic_fixed_point_scale(float q, float *scale, float *unscale)
 {
     /* Q = 2^k, the power of two at or above |q| */
     int k = 0;
@@ -49,7 +49,7 @@ void zpic_fixed_point_scale(float q, int *base, float *unscale)
         frexpf(aq, &k); /* aq = f 2^k, f in [0.5, 1) */
         if (ldexpf(1.0f, k - 1) == aq) k -= 1;
     }
-    *base = 110 + k;
+    *scale = ldexpf(1.0f, 40 - k);
     *unscale = ldexpf(1.0f, k - 40);
 }
 

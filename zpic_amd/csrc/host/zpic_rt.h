@@ -39,7 +39,7 @@ void zpic_tiles_pref(int nx, int ny, int want_x, int want_y, int want_h, zk_tile
 int zpic_push_waves(void);
 int zpic_fixed_deposit(void);
 /* fixed-point deposit constants for particle charge q (see push_core.h) */
-void zpic_fixed_point_scale(float q, int *base, float *unscale);
+void zpic_fixed_point_scale(float q, float *scale, float *unscale);
 size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */
 
 /* GPU timers: spec_time() / emf_time(), and per-kernel-class timers that

@@ -50,26 +50,25 @@ struct LdsCurrent {
 
 /* Fixed-point current sink: contributions are divided by Q (the power of
  * two at or above the particle charge |q|; every contribution is O(|q|)),
- * scaled by 2^40, rounded toward zero to 64-bit integers and summed with
- * integer LDS atomics, so a window's sum is exact and independent of the
- * order of the atomics.  zk_fx(v, base) = v / Q * 2^40 with
- * base = 110 + log2(Q), computed from the float's fields: exact when
- * |v / Q| >= 2^-16, within 2^-40 below; |v / Q| must stay below 2^16. */
-__device__ inline unsigned long long zk_fx(float v, int base)
+ * scaled by 2^40 and truncated to 64-bit integers, which integer LDS
+ * atomics sum exactly, independent of the order of the atomics.
+ * zk_fx(v, scale), scale = 2^40 / Q: x = v * scale is exact (a power of
+ * two); hi = trunc(x / 2^31) and the remainder lo = x - 2^31 hi (exact in
+ * float, |lo| < 2^31) convert with one instruction each, and
+ * trunc(x) = 2^31 hi + trunc(lo).  Valid for |x| < 2^55. */
+__device__ inline unsigned long long zk_fx(float v, float scale)
 {
-    const unsigned u = __float_as_uint(v);
-    const int e = (int)((u >> 23) & 0xffu);
-    const unsigned long long m = (unsigned long long)((u & 0x7fffffu) | 0x800000u);
-    const int sh = e - base;
-    unsigned long long r = sh >= 0 ? (m << (sh < 39 ? sh : 39)) : (sh > -24 ? (m >> -sh) : 0ull);
-    r = e ? r : 0ull;
-    return (u >> 31) ? (0ull - r) : r;
+    const float x = v * scale;
+    const float hi = truncf(x * 4.656612873077393e-10f); /* x / 2^31 */
+    const float lo = x - hi * 2147483648.0f;             /* exact */
+    return (unsigned long long)(((long long)(int)hi << 31) + (long long)(int)lo);
 }
 
 struct LdsCurrentFx {
     unsigned long long *j;
-    int w, n, base;
-    __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], zk_fx(v, base)); }
+    int w, n;
+    float scale;
+    __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], zk_fx(v, scale)); }
 };
 
 struct GlobalCurrent {
@@ -387,7 +386,7 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     lj.n = n;
     if constexpr (FX) {
         lj.j = jx;
-        lj.base = a.fx_base;
+        lj.scale = a.fx_scale;
     } else {
         lj.j = jl;
     }

@@ -73,8 +73,8 @@ typedef struct zk_push_args {
     int global_blocks;          /* blocks of the global (tail) pass      */
     int waves;                  /* tile-pass variant: 0, 2 or 8           */
     int fixed;                  /* 1: fixed-point (int64) current window  */
-    int fx_base;                /* fixed point: 110 + log2(Q), Q = 2^ceil(log2|q|) */
-    float fx_unscale;           /* fixed point: 2^(fx_base - 150)          */
+    float fx_scale;             /* fixed point: 2^40 / Q, Q = 2^ceil(log2|q|) */
+    float fx_unscale;           /* fixed point: Q / 2^40                  */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
 
