@@ -41,6 +41,41 @@ def cases() -> dict:
             "plane_laser": (plane, 40), "odd_grid": (odd, 30)}
 
 
+def report_case():
+    """A deck with non-trivial initial fields (Gaussian laser), a STEP
+    plasma, moving window and external-free particle fields."""
+    cfg = lwfa(120, 16, start=3.0, smooth_level=2)
+    cfg["species"][0]["ppc"] = [2, 2]
+    return cfg
+
+
+PHA_NX = (32, 16)
+PHA_RANGE = ((0.0, 6.0), (-0.05, 0.05))
+
+
+def write_reports(lib, run, outdir: str) -> None:
+    """Every report of the em2d API at the run's current step, into outdir."""
+    cwd = os.getcwd()
+    os.makedirs(outdir, exist_ok=True)
+    os.chdir(outdir)
+    try:
+        for field in range(4):  # EFLD, BFLD, EPART, BPART
+            for fc in range(3):
+                lib.emf_report(C.byref(run.sim.emf), C.c_byte(field), fc)
+        for jc in range(3):
+            lib.current_report(C.byref(run.sim.current), jc)
+        nx = (C.c_int * 2)(*PHA_NX)
+        rng = ((C.c_float * 2) * 2)((C.c_float * 2)(*PHA_RANGE[0]), (C.c_float * 2)(*PHA_RANGE[1]))
+        sp = C.byref(run.species[0])
+        lib.spec_report(sp, 0x3000, None, None)                     # PARTICLES
+        lib.spec_report(sp, 0x2000 + 1 + 4 * 16, nx, rng)           # PHASESPACE(X1, U1)
+        lib.spec_report(sp, 0x2000 + 2 + 6 * 16, nx, rng)           # PHASESPACE(X2, U3)
+        if os.environ.get("ZPIC_REPORT_CHARGE"):
+            lib.spec_report(sp, 0x1000, None, None)                 # CHARGE
+    finally:
+        os.chdir(cwd)
+
+
 def main(name: str) -> int:
     cfg, steps = cases()[name]
     ref = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_em2d.so"))
@@ -58,5 +93,17 @@ def main(name: str) -> int:
     return 0
 
 
+def main_report(outdir: str, steps: int = 0) -> int:
+    ref = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_em2d.so"))
+    run = Em2dRun(ref, report_case(), with_dev=False)
+    run.step(steps)
+    write_reports(ref, run, outdir)
+    run.close()
+    print(json.dumps({"report": outdir}))
+    return 0
+
+
 if __name__ == "__main__":
+    if sys.argv[1] == "report":
+        sys.exit(main_report(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 0))
     sys.exit(main(sys.argv[1]))

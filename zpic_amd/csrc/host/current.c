@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include "zpic_internal.h"
+#include "zpic_zdf.h"
 
 void current_new(t_current *c, int nx[], float box[], float dt)
 {
@@ -169,14 +170,29 @@ void current_sync_host(t_current *c)
 
 const float *zpic_current_J(t_current *c) { return c->dev->J[c->dev->jcur]; }
 
-void current_report(const t_current *c, const int jc)
+void current_report(const t_current *cc, const int jc)
 {
-    static int warned = 0;
-    (void)c;
+    /* em2d/current.c:195-281: one component of J, interior nodes, to
+     * CURRENT/J<jc>-<iter>.zdf */
+    t_current *c = (t_current *)cc;
     if (jc < 0 || jc > 2) {
         fprintf(stderr, "(*error*) Invalid current component (jc) selected, returning\n");
         return;
     }
-    if (!warned) fprintf(stderr, "zpic_em2d: ZDF diagnostic files are not written by this library\n");
-    warned = 1;
+    current_sync_host(c);
+    float *buf = malloc(sizeof(float) * (size_t)c->nx[0] * (size_t)c->nx[1]);
+    for (int j = 0; j < c->nx[1]; j++)
+        for (int i = 0; i < c->nx[0]; i++) {
+            const float3 v = c->J[i + j * c->nrow];
+            buf[i + j * c->nx[0]] = jc == 0 ? v.x : (jc == 1 ? v.y : v.z);
+        }
+    const char comp[] = {'x', 'y', 'z'};
+    char name[16], label[16];
+    snprintf(name, 3, "J%1u", (unsigned)jc);
+    snprintf(label, 4, "J_%c", comp[jc]);
+    const zdf_axis axis[2] = {{"x", "x", "c/\\omega_p", 0.0, c->box[0]}, {"y", "y", "c/\\omega_p", 0.0, c->box[1]}};
+    const zdf_grid g = {name, label, "e \\omega_p^2 / c", 2, {(uint64_t)c->nx[0], (uint64_t)c->nx[1], 0}, axis};
+    const zdf_iter it = {"ITERATION", c->iter, c->iter * c->dt, "1/\\omega_p"};
+    zpic_zdf_save_grid(buf, &g, &it, "CURRENT");
+    free(buf);
 }

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include "zpic_internal.h"
+#include "zpic_zdf.h"
 
 void emf_new(t_emf *emf, int nx[], float box[], const float dt)
 {
@@ -458,15 +459,53 @@ void emf_get_energy(const t_emf *cemf, double energy[])
     zpic_d2h(energy, d->energy6, sizeof(double) * 6);
 }
 
-void emf_report(const t_emf *emf, const char field, const int fc)
+void emf_report(const t_emf *cemf, const char field, const int fc)
 {
-    static int warned = 0;
-    (void)emf;
-    (void)field;
+    /* em2d/emf.c:368-480: one component of E, B or the particle fields,
+     * interior nodes, to EMF/<name>-<iter>.zdf */
+    t_emf *emf = (t_emf *)cemf;
     if (fc < 0 || fc > 2) {
         fprintf(stderr, "(*error*) Invalid field component (fc) selected, returning\n");
         return;
     }
-    if (!warned) fprintf(stderr, "zpic_em2d: ZDF diagnostic files are not written by this library\n");
-    warned = 1;
+    const char comp[] = {'x', 'y', 'z'};
+    char name[16], label[16];
+    const float3 *f;
+    emf_sync_host(emf);
+    switch (field) {
+    case EFLD:
+        f = emf->E;
+        snprintf(name, 16, "E%1d", fc);
+        snprintf(label, 16, "E_%c", comp[fc]);
+        break;
+    case BFLD:
+        f = emf->B;
+        snprintf(name, 16, "B%1d", fc);
+        snprintf(label, 16, "B_%c", comp[fc]);
+        break;
+    case EPART:
+        f = emf->E_part;
+        snprintf(name, 16, "E%1d-part", fc);
+        snprintf(label, 16, "E_{%cp}", comp[fc]);
+        break;
+    case BPART:
+        f = emf->B_part;
+        snprintf(name, 16, "B%1d-part", fc);
+        snprintf(label, 16, "B_{%cp}", comp[fc]);
+        break;
+    default:
+        fprintf(stderr, "Invalid field type selected, returning\n");
+        return;
+    }
+    float *buf = malloc(sizeof(float) * (size_t)emf->nx[0] * (size_t)emf->nx[1]);
+    for (int j = 0; j < emf->nx[1]; j++)
+        for (int i = 0; i < emf->nx[0]; i++) {
+            const float3 v = f[i + j * emf->nrow];
+            buf[i + j * emf->nx[0]] = fc == 0 ? v.x : (fc == 1 ? v.y : v.z);
+        }
+    const zdf_axis axis[2] = {{"x", "x", "c/\\omega_p", 0.0, emf->box[0]}, {"y", "y", "c/\\omega_p", 0.0, emf->box[1]}};
+    const zdf_grid g = {name, label, "m_e c \\omega_p e^{-1}", 2, {(uint64_t)emf->nx[0], (uint64_t)emf->nx[1], 0}, axis};
+    const zdf_iter it = {"ITERATION", emf->iter, emf->iter * emf->dt, "1/\\omega_p"};
+    zpic_zdf_save_grid(buf, &g, &it, "EMF");
+    free(buf);
 }

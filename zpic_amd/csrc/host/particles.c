@@ -17,6 +17,7 @@
 #include "random.h"
 #include "timer.h"
 #include "zpic_internal.h"
+#include "zpic_zdf.h"
 
 /* all live species, for the process-wide push counter; spec_new may run
  * on several host threads at once (independent simulations initialised
@@ -751,13 +752,95 @@ void spec_deposit_pha(const t_species *cs, const int rep_type, const int pha_nx[
     }
 }
 
-void spec_report(const t_species *s, const int rep_type, const int pha_nx[], const float pha_range[][2])
+/* em2d/particles.c:1339-1416: positions (from the box corner) and
+ * momenta of every particle, to PARTICLES/<name>/particles-<name>-<iter>.zdf;
+ * the particles are in this library's tile order */
+static void rep_particles(t_species *s)
 {
-    static int warned = 0;
-    (void)s;
-    (void)rep_type;
-    (void)pha_nx;
-    (void)pha_range;
-    if (!warned) fprintf(stderr, "zpic_em2d: ZDF diagnostic files are not written by this library\n");
-    warned = 1;
+    spec_sync_host(s);
+    const int np = s->np > 0 ? s->np : 0;
+    float *d[5];
+    for (int k = 0; k < 5; k++) d[k] = malloc(sizeof(float) * (size_t)(np > 0 ? np : 1));
+    for (int i = 0; i < np; i++) {
+        const t_part *p = &s->part[i];
+        d[0][i] = (s->n_move + p->ix + p->x) * s->dx[0];
+        d[1][i] = (p->iy + p->y) * s->dx[1];
+        d[2][i] = p->ux;
+        d[3][i] = p->uy;
+        d[4][i] = p->uz;
+    }
+    static const char *const quants[] = {"x", "y", "ux", "uy", "uz"};
+    static const char *const qlabels[] = {"x", "y", "u_x", "u_y", "u_z"};
+    static const char *const qunits[] = {"c/\\omega_p", "c/\\omega_p", "c", "c", "c"};
+    const zdf_iter it = {"ITERATION", s->iter, s->iter * s->dt, "1/\\omega_p"};
+    char path[1024];
+    snprintf(path, 1024, "PARTICLES/%s", s->name);
+    zpic_zdf_save_particles(s->name, s->name, (uint64_t)np, 5, quants, qlabels, qunits, (const float *const *)d, &it,
+                            path);
+    for (int k = 0; k < 5; k++) free(d[k]);
+}
+
+/* em2d/particles.c:1423-1497: charge density, interior nodes, to
+ * CHARGE/<name>/<name>-charge-<iter>.zdf */
+static void rep_charge(t_species *s)
+{
+    const size_t n1 = (size_t)(s->nx[0] + 1) * (size_t)(s->nx[1] + 1);
+    float *charge = calloc(n1, sizeof(float));
+    spec_deposit_charge(s, charge);
+    float *buf = malloc(sizeof(float) * (size_t)s->nx[0] * (size_t)s->nx[1]);
+    for (int j = 0; j < s->nx[1]; j++)
+        for (int i = 0; i < s->nx[0]; i++) buf[i + j * s->nx[0]] = charge[i + j * (s->nx[0] + 1)];
+    free(charge);
+    const zdf_axis axis[2] = {{"x", "x", "c/\\omega_p", s->n_move * s->dx[0], s->box[0] + s->n_move * s->dx[0]},
+                              {"y", "y", "c/\\omega_p", 0.0, s->box[1]}};
+    char name[128], label[128];
+    snprintf(name, 128, "%s-charge", s->name);
+    snprintf(label, 128, "%s \\rho", s->name);
+    const zdf_grid g = {name, label, "n_e", 2, {(uint64_t)s->nx[0], (uint64_t)s->nx[1], 0}, axis};
+    const zdf_iter it = {"ITERATION", s->iter, s->iter * s->dt, "1/\\omega_p"};
+    char path[1024];
+    snprintf(path, 1024, "CHARGE/%s", s->name);
+    zpic_zdf_save_grid(buf, &g, &it, path);
+    free(buf);
+}
+
+/* em2d/particles.c:1640-1711: phase-space density to
+ * PHASESPACE/<name>/<name>-<q1><q2>-<iter>.zdf */
+static void rep_pha(t_species *s, int rep_type, const int pha_nx[], const float pha_range[][2])
+{
+    float *buf = calloc((size_t)pha_nx[0] * (size_t)pha_nx[1], sizeof(float));
+    spec_deposit_pha(s, rep_type, pha_nx, pha_range, buf);
+    const int q1 = rep_type & 0x000F, q2 = (rep_type & 0x00F0) >> 4;
+    static const char *const ax_name[] = {"x1", "x2", "x3", "u1", "u2", "u3"};
+    static const char *const ax_label[] = {"x", "y", "z", "u_x", "u_y", "u_z"};
+    const char *u1 = (q1 == X1 || q1 == X2) ? "c/\\omega_p" : ((q1 >= U1 && q1 <= U3) ? "m_e c" : "");
+    const char *u2 = (q2 == X1 || q2 == X2) ? "c/\\omega_p" : ((q2 >= U1 && q2 <= U3) ? "m_e c" : "");
+    const zdf_axis axis[2] = {{ax_name[q1 - 1], ax_label[q1 - 1], u1, pha_range[0][0], pha_range[0][1]},
+                              {ax_name[q2 - 1], ax_label[q2 - 1], u2, pha_range[1][0], pha_range[1][1]}};
+    char name[64], label[64];
+    snprintf(name, 64, "%s-%s%s", s->name, ax_name[q1 - 1], ax_name[q2 - 1]);
+    snprintf(label, 64, "%s %s-%s", s->name, ax_label[q1 - 1], ax_label[q2 - 1]);
+    const zdf_grid g = {name, label, "a.u.", 2, {(uint64_t)pha_nx[0], (uint64_t)pha_nx[1], 0}, axis};
+    const zdf_iter it = {"ITERATION", s->iter, s->iter * s->dt, "1/\\omega_p"};
+    char path[1024];
+    snprintf(path, 1024, "PHASESPACE/%s", s->name);
+    zpic_zdf_save_grid(buf, &g, &it, path);
+    free(buf);
+}
+
+void spec_report(const t_species *cs, const int rep_type, const int pha_nx[], const float pha_range[][2])
+{
+    /* em2d/particles.c:1724-1740 */
+    t_species *s = (t_species *)cs;
+    switch (rep_type & 0xF000) {
+    case CHARGE:
+        rep_charge(s);
+        break;
+    case PHA:
+        rep_pha(s, rep_type, pha_nx, pha_range);
+        break;
+    case PARTICLES:
+        rep_particles(s);
+        break;
+    }
 }
