@@ -252,18 +252,24 @@ static double b_energy(t_simulation *sim)
     return e[3] + e[4] + e[5];
 }
 
-/* sim_iter with the species' sorts switched off (the checkpoints sort) */
+/* sim_iter with the species' sorts switched off (the checkpoints sort),
+ * so particles keep their slots through a segment */
 static void step_unsorted(zpic_diff *D)
 {
     t_simulation *sim = D->sim;
-    int saved[64];
+    int saved[64], saved_auto[64];
     const int ns = D->ns < 64 ? D->ns : 64;
     for (int k = 0; k < ns; k++) {
         saved[k] = sim->species[k].n_sort;
+        saved_auto[k] = sim->species[k].dev->auto_sort;
         sim->species[k].n_sort = 0;
+        sim->species[k].dev->auto_sort = 0;
     }
     sim_iter(sim);
-    for (int k = 0; k < ns; k++) sim->species[k].n_sort = saved[k];
+    for (int k = 0; k < ns; k++) {
+        sim->species[k].n_sort = saved[k];
+        sim->species[k].dev->auto_sort = saved_auto[k];
+    }
 }
 
 double zpic_diff_forward(zpic_diff *D)

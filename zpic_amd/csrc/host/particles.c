@@ -259,6 +259,7 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
     struct zpic_dev_species *d = calloc(1, sizeof(*d));
     d->host_dirty = 1;
     d->host_valid = 1;
+    d->auto_sort = 1;
     d->owner = s;
     pthread_mutex_lock(&g_live_lock);
     d->next_live = g_live;
@@ -432,6 +433,7 @@ static void sort_now(t_species *s, int spread)
     d->cur = other;
     d->tail_upper = 0;
     d->moved_since_sort = 0;
+    d->steps_since_sort = 0;
 }
 
 void zpic_spec_sort(t_species *s) { sort_now(s, 0); }
@@ -656,7 +658,20 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0],
                             (double)s->dx[1],
                             zpic_stream()));
-    if ((s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0) sort_now(s, 0);
+    /* the species' schedule (em2d/particles.c:1262-1264); particles that
+     * left their tile windows take the slow global path, so the tiles are
+     * also re-sorted early when that path carries too many pushes (the
+     * order of the particles is not part of the results) */
+    int sort = (s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0;
+    d->steps_since_sort++;
+    if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
+        zk_spec_ctr c;
+        zpic_d2h(&c, d->ctr, sizeof(c));
+        const long long novf = (long long)(unsigned)c.n_overflow - d->ovf_seen;
+        d->ovf_seen = (long long)(unsigned)c.n_overflow;
+        if (novf * 1000 > (long long)ZPIC_SORT_OVF_PERMILLE * ZPIC_SORT_PROBE * (long long)c.np_total) sort = 1;
+    }
+    if (sort) sort_now(s, 0);
     zpic_timer_end(ZPIC_T_SPEC);
     d->host_valid = 0;
 }

@@ -19,6 +19,11 @@
 
 /* blocks of the global (out-of-window and tail) push pass */
 #define ZPIC_GLOBAL_BLOCKS 512
+/* early tile sort: every ZPIC_SORT_PROBE pushes the overflow counter is
+ * read; if more than ZPIC_SORT_OVF_PERMILLE per mille of the pushes since
+ * the previous probe took the global path, the tiles are re-sorted */
+#define ZPIC_SORT_PROBE 4
+#define ZPIC_SORT_OVF_PERMILLE 5
 
 struct zpic_dev_species {
     int ready;            /* device buffers allocated and filled */
@@ -41,6 +46,9 @@ struct zpic_dev_species {
     int np_upper;         /* upper bound of slots in use */
     int tail_upper;       /* upper bound of unsorted slots */
     int moved_since_sort; /* window moves since the last sort */
+    int steps_since_sort; /* pushes since the last tile sort */
+    int auto_sort;        /* 1: also sort early when many particles left their tiles */
+    long long ovf_seen;   /* overflow count at the last probe */
     unsigned long long npush_retired; /* pushes counted before buffers were freed */
     int64_t *host_ids;    /* ids in host-mirror order (when tracked) */
     int host_ids_cap;
