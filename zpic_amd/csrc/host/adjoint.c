@@ -79,15 +79,7 @@ static void parts_free(zk_parts *p)
 
 static void parts_copy(const zk_parts *dst, const zk_parts *src, int n)
 {
-    const size_t b = 4 * (size_t)n;
-    zpic_d2d_async(dst->ix, src->ix, b);
-    zpic_d2d_async(dst->iy, src->iy, b);
-    zpic_d2d_async(dst->x, src->x, b);
-    zpic_d2d_async(dst->y, src->y, b);
-    zpic_d2d_async(dst->ux, src->ux, b);
-    zpic_d2d_async(dst->uy, src->uy, b);
-    zpic_d2d_async(dst->uz, src->uz, b);
-    if (dst->id && src->id) zpic_d2d_async(dst->id, src->id, 2 * b);
+    ZCHECK(zk_copy_parts(dst, src, n, zpic_stream()));
 }
 
 static void vec_alloc(zk_adj_vec *v, int n)
@@ -214,9 +206,7 @@ static void save_state(zpic_diff *D, int c)
             zpic_fatal("zpic_diff: species '%s' changed its particle count", s->name);
     }
     struct zpic_dev_emf *de = sim->emf.dev;
-    const size_t gb = sizeof(float) * 3 * D->nodes;
-    zpic_d2d_async(D->cE[c], de->E[de->cur], gb);
-    zpic_d2d_async(D->cB[c], de->B[de->cur], gb);
+    ZCHECK(zk_copy_pair(D->cE[c], de->E[de->cur], D->cB[c], de->B[de->cur], 3 * D->nodes, zpic_stream()));
 }
 
 /* put slot c back into the simulation, at iteration `iter` */
@@ -234,9 +224,7 @@ static void restore_state(zpic_diff *D, int c, int iter)
         s->iter = iter;
     }
     struct zpic_dev_emf *de = sim->emf.dev;
-    const size_t gb = sizeof(float) * 3 * D->nodes;
-    zpic_d2d_async(de->E[de->cur], D->cE[c], gb);
-    zpic_d2d_async(de->B[de->cur], D->cB[c], gb);
+    ZCHECK(zk_copy_pair(de->E[de->cur], D->cE[c], de->B[de->cur], D->cB[c], 3 * D->nodes, zpic_stream()));
     de->host_valid = 0;
     sim->emf.iter = iter;
     sim->current.iter = iter;
@@ -314,12 +302,10 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
                 struct zpic_dev_species *d = sim->species[k].dev;
                 parts_copy(&D->rp[(size_t)(n - s0) * D->ns + k], &d->set[d->cur], D->np[k]);
             }
-            zpic_d2d_async(D->rE[n - s0], de->E[de->cur], gb);
-            zpic_d2d_async(D->rB[n - s0], de->B[de->cur], gb);
+            ZCHECK(zk_copy_pair(D->rE[n - s0], de->E[de->cur], D->rB[n - s0], de->B[de->cur], 3 * D->nodes, st));
             step_unsorted(D);
         }
-        zpic_d2d_async(D->rE[s1 - s0], de->E[de->cur], gb);
-        zpic_d2d_async(D->rB[s1 - s0], de->B[de->cur], gb);
+        ZCHECK(zk_copy_pair(D->rE[s1 - s0], de->E[de->cur], D->rB[s1 - s0], de->B[de->cur], 3 * D->nodes, st));
         /* particle adjoints: from the order of the next checkpoint (sorted
          * after this segment) to this segment's order */
         if (c < D->nck - 1) {

@@ -400,7 +400,6 @@ __device__ inline void deposit_adj(const S &src, int ix, int iy, int di, int dj,
     s0.iy = iy;
     s1 = s0;
     s2 = s0;
-    const SegF init = s0;
     float fx = 0.0f, fy = 0.0f;
     int n = 1, ycase = 0;
     if (di != 0) {
@@ -455,15 +454,16 @@ __device__ inline void deposit_adj(const S &src, int ix, int iy, int di, int dj,
     }
     SegF A = a0;
     if (di != 0) {
-        /* adjoint of the x cut of init into (s0, s1) */
+        /* adjoint of the x cut of the initial segment (x0, y0, mx, my,
+         * x0 + mx, y0 + my, qvz / 2) into (s0, s1) */
+        const float q0 = qvz * 0.5f;
         const float lyc = a1.y0 + a0.y1;
-        const float lf = (a1.dx - a0.dx) * init.dx + (a1.dy - a0.dy) * init.dy + (a1.q - a0.q) * init.q -
-                         lyc * init.dy;
+        const float lf = (a1.dx - a0.dx) * mx + (a1.dy - a0.dy) * my + (a1.q - a0.q) * q0 - lyc * my;
         A.x0 = a0.x0;
         A.y0 = a0.y0 + lyc;
-        A.dx = a1.dx * fx + a0.dx * (1.0f - fx) - lf * fx / init.dx;
+        A.dx = a1.dx * fx + a0.dx * (1.0f - fx) - lf * fx / mx;
         A.dy = a1.dy * fx + (a0.dy + lyc) * (1.0f - fx);
-        A.x1 = a1.x1 + lf / init.dx;
+        A.x1 = a1.x1 + lf / mx;
         A.y1 = a1.y1;
         A.q = a1.q * fx + a0.q * (1.0f - fx);
     }
@@ -488,13 +488,13 @@ __device__ inline void adjoint_one(const S &src, int ci, int cj, float x, float 
                                    float l[5], const AdjConsts &k)
 {
     const Stagger st(ci, cj, x, y);
-    float f[6], fa[6], fb[6];
+    float f[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
         int ia, jb;
-        float wa, wb;
+        float wa, wb, da, db;
         st.of(c, ia, jb, wa, wb);
-        f[c] = interp(src, c, ia, jb, wa, wb, fa[c], fb[c]);
+        f[c] = interp(src, c, ia, jb, wa, wb, da, db);
     }
     /* forward Boris (push_one of push_core.h) */
     const float e0 = f[0] * k.tem, e1 = f[1] * k.tem, e2 = f[2] * k.tem;
@@ -561,14 +561,17 @@ __device__ inline void adjoint_one(const S &src, int ci, int cj, float x, float 
     le2 += laz;
     const float lf[6] = {le0 * k.tem, le1 * k.tem, le2 * k.tem, lb0, lb1, lb2};
     float gxs = 0.0f, gys = 0.0f;
+    /* the derivatives of the gather are formed again here, which keeps
+     * them out of the registers live across the deposit adjoint */
 #pragma unroll
     for (int c = 0; c < 6; c++) {
         int ia, jb;
-        float wa, wb;
+        float wa, wb, da, db;
         st.of(c, ia, jb, wa, wb);
+        interp(src, c, ia, jb, wa, wb, da, db);
         interp_scatter(src, c, ia, jb, wa, wb, lf[c]);
-        gxs += lf[c] * fa[c];
-        gys += lf[c] * fb[c];
+        gxs += lf[c] * da;
+        gys += lf[c] * db;
     }
     l[0] += gx + gxs;
     l[1] += gy + gys;
@@ -726,9 +729,50 @@ __global__ void adj_by_id_kernel(zk_adj_vec v, const int64_t *__restrict__ ids, 
     o[4] = v.luz[i];
 }
 
+/* copies of particle sets and grid pairs for checkpoints and records:
+ * one launch instead of one runtime copy per array */
+__global__ void copy_parts_kernel(zk_parts dst, zk_parts src, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        dst.ix[i] = src.ix[i];
+        dst.iy[i] = src.iy[i];
+        dst.x[i] = src.x[i];
+        dst.y[i] = src.y[i];
+        dst.ux[i] = src.ux[i];
+        dst.uy[i] = src.uy[i];
+        dst.uz[i] = src.uz[i];
+        if (dst.id && src.id) dst.id[i] = src.id[i];
+    }
+}
+
+__global__ void copy_pair_kernel(float *__restrict__ a, const float *__restrict__ sa, float *__restrict__ b,
+                                 const float *__restrict__ sb, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        a[i] = sa[i];
+        b[i] = sb[i];
+    }
+}
+
 unsigned grid_of(size_t n) { return (unsigned)((n + 255) / 256); }
 
 } // namespace
+
+extern "C" int zk_copy_parts(const zk_parts *dst, const zk_parts *src, int n, zk_stream_t s)
+{
+    if (n <= 0) return 0;
+    const unsigned g = grid_of((size_t)n) < 4096u ? grid_of((size_t)n) : 4096u;
+    hipLaunchKernelGGL(copy_parts_kernel, dim3(g), dim3(256), 0, s, *dst, *src, n);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_copy_pair(float *a, const float *sa, float *b, const float *sb, size_t n, zk_stream_t s)
+{
+    if (n == 0) return 0;
+    const unsigned g = grid_of(n) < 4096u ? grid_of(n) : 4096u;
+    hipLaunchKernelGGL(copy_pair_kernel, dim3(g), dim3(256), 0, s, a, sa, b, sb, n);
+    ZK_LAUNCH_CHECK();
+}
 
 extern "C" int zk_adj_loss(float *lB, const float *B, int nx, int ny, int nrow, float scale, zk_stream_t s)
 {

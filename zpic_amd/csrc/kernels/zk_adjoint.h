@@ -36,6 +36,10 @@ typedef struct zk_adj_push_args {
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_adj_push_args;
 
+/* dst = src for the seven particle arrays (and ids when both have them) */
+int zk_copy_parts(const zk_parts *dst, const zk_parts *src, int n, zk_stream_t s);
+/* a = sa and b = sb, n floats each */
+int zk_copy_pair(float *a, const float *sa, float *b, const float *sb, size_t n, zk_stream_t s);
 /* lB(interior) += scale * B(interior) */
 int zk_adj_loss(float *lB, const float *B, int nx, int ny, int nrow, float scale, zk_stream_t s);
 /* Reverse of zk_yee + zk_field_guards (periodic): on entry lE, lB hold the
