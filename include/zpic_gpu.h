@@ -138,6 +138,12 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int steps, int ckpt, int loss_first,
 double zpic_diff_forward(zpic_diff *run);
 void zpic_diff_backward(zpic_diff *run, float *dE, float *dB, float *const *dpart);
 void zpic_diff_delete(zpic_diff *run);
+/* Before zpic_diff_forward: on = 1 makes the forward run keep the state at
+ * the start of every step (28 bytes per particle and step, plus E and B
+ * per step) so the backward pass runs no segment twice; on = 0 (default)
+ * keeps checkpoints only and re-runs each segment during the backward
+ * pass.  The gradients are the same either way. */
+void zpic_diff_record_all(zpic_diff *run, int on);
 
 #ifdef __cplusplus
 }

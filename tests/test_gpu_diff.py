@@ -176,3 +176,31 @@ def test_adjoint_matches_central_differences_of_gpu_runs():
     assert abs(g_uz - fd_uz) <= 1e-3 * abs(fd_uz)
     assert abs(g_uth - fd_uth) <= 2e-2 * abs(fd_uth)
     assert abs(g_s - fd_s) <= 1e-2 * abs(fd_s)
+
+
+def test_recorded_trajectory_gives_the_checkpointed_gradients():
+    """zpic_diff_record_all: the forward run keeps every step and the
+    backward pass runs no segment twice; the adjoints equal those of the
+    checkpointed run up to the order of the float atomics (the reverse
+    step's LDS sums and the deposit of out-of-window particles)."""
+    import dataclasses
+
+    cfg = WeibelConfig(nx=64, ny=64, ppc=4, steps=60, ckpt=20, loss_first=30, loss_last=60)
+    seed = 1e-3 * np.random.default_rng(3).standard_normal((16, 16))
+    out = {}
+    for record in (False, True):
+        run = WeibelRun(dataclasses.replace(cfg, record=record), 0.6, 0.1, seed)
+        try:
+            print(f"diff: 64x64 record={record}", flush=True)
+            L = run.forward()
+            out[record] = (L, run.backward())
+        finally:
+            run.close()
+    (L0, (dE0, dB0, p0)), (L1, (dE1, dB1, p1)) = out[False], out[True]
+    print(f"loss {L0:.9e} {L1:.9e}; dE {rel(dE1, dE0):.2e} dB {rel(dB1, dB0):.2e} "
+          f"parts {[rel(p1[k], p0[k]) for k in range(2)]}", flush=True)
+    assert abs(L1 - L0) <= 1e-6 * abs(L0)
+    assert rel(dE1, dE0) <= 1e-4
+    assert rel(dB1, dB0) <= 1e-4
+    for k in range(2):
+        assert rel(p1[k], p0[k]) <= 1e-4

@@ -13,6 +13,12 @@
  * the particle adjoints are carried across the checkpoint's sort through
  * the ids.
  *
+ * With recording on (zpic_diff_record_all) the forward run itself keeps
+ * the state at the start of every step, and the backward pass goes
+ * straight to the reverse steps: no segment is run twice, at the cost of
+ * the memory of the whole trajectory (28 bytes per particle and step plus
+ * two grids per step).
+ *
  * The loss is  L = sum over steps n in [loss_first, loss_last] of
  * dt * (magnetic terms of emf_get_energy at step n), FP64; its gradient
  * with respect to the grid B is dt * dx * dy * B on interior nodes.
@@ -34,20 +40,26 @@ struct zpic_diff {
     t_simulation *sim;
     int nsteps, ckpt, nck, loss_first, loss_last;
     int ns;
+    int record;        /* 1: the forward run records every step */
+    int nrec;          /* particle-state slots per species in rp */
     size_t nodes;
     int *np;           /* particles per species */
     spec_ckpt *cs;     /* [(nck + 1) * ns]: checkpoints, then the final state */
     float **cE, **cB;  /* [nck + 1] */
-    zk_parts *rp;      /* [ckpt * ns]: particle state at the start of each step of a segment */
-    float **rE, **rB;  /* [ckpt + 1]: fields at the start of each step, then at the end */
+    zk_parts *rp;      /* [nrec * ns]: particle state at the start of each step of a segment
+                          (of the run when recording) */
+    float **rE, **rB;  /* [nrec + 1]: fields at the start of each step, then at the end */
     zk_adj_vec *lam, *lam2; /* per species */
     int **pos;
     float *lE, *lB, *lJ, *aE, *aB, *Lam, *region;
-    int *ovf_cnt;
+    int splits;        /* workgroups per tile in the reverse push */
     float *out;        /* device staging of one species' adjoints by id */
     int forward_done;
     int iter0;
 };
+
+/* workgroups the reverse push aims for (tiles times splits) */
+#define ZPIC_ADJ_BLOCKS 1024
 
 #define CS(D, c, k) (&(D)->cs[(size_t)(c) * (D)->ns + (k)])
 
@@ -111,6 +123,41 @@ static void vec_zero(const zk_adj_vec *v, int n)
     ZCHECK(zk_fill_f32(v->luz, (size_t)n, 0.0f, s));
 }
 
+/* record slots for nrec steps */
+static void records_alloc(zpic_diff *D, int nrec)
+{
+    const size_t gb = sizeof(float) * 3 * D->nodes;
+    D->nrec = nrec;
+    D->rp = calloc((size_t)nrec * (D->ns > 0 ? D->ns : 1), sizeof(zk_parts));
+    for (int n = 0; n < nrec; n++)
+        for (int k = 0; k < D->ns; k++) parts_alloc(&D->rp[(size_t)n * D->ns + k], D->np[k], 0);
+    D->rE = calloc((size_t)nrec + 1, sizeof(float *));
+    D->rB = calloc((size_t)nrec + 1, sizeof(float *));
+    for (int n = 0; n <= nrec; n++) {
+        D->rE[n] = zpic_dmalloc(gb);
+        D->rB[n] = zpic_dmalloc(gb);
+    }
+}
+
+static void records_free(zpic_diff *D)
+{
+    for (int n = 0; n < D->nrec; n++)
+        for (int k = 0; k < D->ns; k++) parts_free(&D->rp[(size_t)n * D->ns + k]);
+    for (int n = 0; n <= D->nrec; n++) {
+        zpic_dfree(D->rE[n]);
+        zpic_dfree(D->rB[n]);
+    }
+    free(D->rp);
+    free(D->rE);
+    free(D->rB);
+    D->rp = NULL;
+    D->rE = D->rB = NULL;
+    D->nrec = 0;
+}
+
+/* record slot of step n of the segment starting at s0 */
+static int rslot(const zpic_diff *D, int n, int s0) { return D->record ? n : n - s0; }
+
 zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first, int loss_last)
 {
     if (!sim || !sim->emf.dev) zpic_fatal("zpic_diff_new: simulation was not created with sim_new");
@@ -159,15 +206,7 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
             CS(D, c, k)->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(t->ntiles + 2));
         }
     }
-    D->rp = calloc((size_t)D->ckpt * (D->ns > 0 ? D->ns : 1), sizeof(zk_parts));
-    for (int n = 0; n < D->ckpt; n++)
-        for (int k = 0; k < D->ns; k++) parts_alloc(&D->rp[(size_t)n * D->ns + k], D->np[k], 0);
-    D->rE = calloc((size_t)D->ckpt + 1, sizeof(float *));
-    D->rB = calloc((size_t)D->ckpt + 1, sizeof(float *));
-    for (int n = 0; n <= D->ckpt; n++) {
-        D->rE[n] = zpic_dmalloc(gb);
-        D->rB[n] = zpic_dmalloc(gb);
-    }
+    records_alloc(D, D->ckpt < D->nsteps ? D->ckpt : D->nsteps);
     D->lam = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(zk_adj_vec));
     D->lam2 = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(zk_adj_vec));
     D->pos = calloc((size_t)(D->ns > 0 ? D->ns : 1), sizeof(int *));
@@ -184,8 +223,10 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
     D->aE = zpic_dmalloc(gb);
     D->aB = zpic_dmalloc(gb);
     D->Lam = zpic_dmalloc(gb);
-    D->region = zpic_dmalloc(sizeof(float) * 6 * (size_t)t->WX * t->WY * t->ntiles);
-    D->ovf_cnt = zpic_dmalloc(sizeof(int) * (size_t)(t->ntiles > 0 ? t->ntiles : 1));
+    /* enough workgroups to fill the GPU on small grids (a few per CU) */
+    D->splits = t->ntiles > 0 ? (ZPIC_ADJ_BLOCKS + t->ntiles - 1) / t->ntiles : 1;
+    if (D->splits > 8) D->splits = 8;
+    D->region = zpic_dmalloc(sizeof(float) * 6 * (size_t)t->WX * t->WY * t->ntiles * D->splits);
     D->out = zpic_dmalloc(sizeof(float) * 5 * (size_t)npmax);
     return D;
 }
@@ -231,6 +272,38 @@ static void restore_state(zpic_diff *D, int c, int iter)
     sim->current.dev->host_valid = 0;
 }
 
+void zpic_diff_record_all(zpic_diff *D, int on)
+{
+    if (!D) zpic_fatal("zpic_diff_record_all: no run");
+    if (D->forward_done) zpic_fatal("zpic_diff_record_all: call before zpic_diff_forward");
+    on = on ? 1 : 0;
+    if (on == D->record) return;
+    zpic_sync();
+    records_free(D);
+    D->record = on;
+    records_alloc(D, on ? D->nsteps : (D->ckpt < D->nsteps ? D->ckpt : D->nsteps));
+}
+
+/* the state at the start of step n (segment start s0) into its record slot */
+static void record_state(zpic_diff *D, int n, int s0)
+{
+    struct zpic_dev_emf *de = D->sim->emf.dev;
+    const int r = rslot(D, n, s0);
+    for (int k = 0; k < D->ns; k++) {
+        struct zpic_dev_species *d = D->sim->species[k].dev;
+        parts_copy(&D->rp[(size_t)r * D->ns + k], &d->set[d->cur], D->np[k]);
+    }
+    ZCHECK(zk_copy_pair(D->rE[r], de->E[de->cur], D->rB[r], de->B[de->cur], 3 * D->nodes, zpic_stream()));
+}
+
+/* the fields at the end of the segment [s0, s1) into their record slot */
+static void record_fields(zpic_diff *D, int s1, int s0)
+{
+    struct zpic_dev_emf *de = D->sim->emf.dev;
+    const int r = rslot(D, s1, s0);
+    ZCHECK(zk_copy_pair(D->rE[r], de->E[de->cur], D->rB[r], de->B[de->cur], 3 * D->nodes, zpic_stream()));
+}
+
 static int in_loss(const zpic_diff *D, int n) { return n >= D->loss_first && n <= D->loss_last; }
 
 static double b_energy(t_simulation *sim)
@@ -271,9 +344,11 @@ double zpic_diff_forward(zpic_diff *D)
     if (in_loss(D, 0)) loss += dt * b_energy(sim);
     for (int n = 0; n < D->nsteps; n++) {
         if (n % D->ckpt == 0) save_state(D, n / D->ckpt);
+        if (D->record) record_state(D, n, 0);
         step_unsorted(D);
         if (in_loss(D, n + 1)) loss += dt * b_energy(sim);
     }
+    if (D->record) record_fields(D, D->nsteps, 0);
     save_state(D, D->nck);
     D->forward_done = 1;
     return loss;
@@ -291,21 +366,18 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
     ZCHECK(zk_fill_f32(D->lE, 3 * D->nodes, 0.0f, st));
     ZCHECK(zk_fill_f32(D->lB, 3 * D->nodes, 0.0f, st));
     for (int k = 0; k < D->ns; k++) vec_zero(&D->lam[k], D->np[k]);
-    struct zpic_dev_emf *de = emf->dev;
 
     for (int c = D->nck - 1; c >= 0; c--) {
         const int s0 = c * D->ckpt, s1 = (s0 + D->ckpt < D->nsteps) ? s0 + D->ckpt : D->nsteps;
-        /* re-run the segment, recording the state at the start of each step */
-        restore_state(D, c, D->iter0 + s0);
-        for (int n = s0; n < s1; n++) {
-            for (int k = 0; k < D->ns; k++) {
-                struct zpic_dev_species *d = sim->species[k].dev;
-                parts_copy(&D->rp[(size_t)(n - s0) * D->ns + k], &d->set[d->cur], D->np[k]);
+        if (!D->record) {
+            /* re-run the segment, recording the state at the start of each step */
+            restore_state(D, c, D->iter0 + s0);
+            for (int n = s0; n < s1; n++) {
+                record_state(D, n, s0);
+                step_unsorted(D);
             }
-            ZCHECK(zk_copy_pair(D->rE[n - s0], de->E[de->cur], D->rB[n - s0], de->B[de->cur], 3 * D->nodes, st));
-            step_unsorted(D);
+            record_fields(D, s1, s0);
         }
-        ZCHECK(zk_copy_pair(D->rE[s1 - s0], de->E[de->cur], D->rB[s1 - s0], de->B[de->cur], 3 * D->nodes, st));
         /* particle adjoints: from the order of the next checkpoint (sorted
          * after this segment) to this segment's order */
         if (c < D->nck - 1) {
@@ -318,7 +390,8 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
             }
         }
         for (int n = s1 - 1; n >= s0; n--) {
-            if (in_loss(D, n + 1)) ZCHECK(zk_adj_loss(D->lB, D->rB[n + 1 - s0], nx, ny, nrow, scale, st));
+            const int r = rslot(D, n, s0);
+            if (in_loss(D, n + 1)) ZCHECK(zk_adj_loss(D->lB, D->rB[r + 1], nx, ny, nrow, scale, st));
             ZCHECK(zk_adj_fields(D->lE, D->lB, D->lJ, D->aE, D->aB, nx, ny, nrow, sim->dt, emf->dx[0], emf->dx[1],
                                  st));
             ZCHECK(zk_adj_images(D->lJ, D->Lam, nx, ny, nrow, st));
@@ -327,10 +400,10 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
                 struct zpic_dev_species *d = s->dev;
                 zk_adj_push_args a;
                 memset(&a, 0, sizeof(a));
-                a.p = D->rp[(size_t)(n - s0) * D->ns + k];
+                a.p = D->rp[(size_t)r * D->ns + k];
                 a.tile_off = CS(D, c, k)->tile_off;
-                a.E = D->rE[n - s0];
-                a.B = D->rB[n - s0];
+                a.E = D->rE[r];
+                a.B = D->rB[r];
                 a.lam = D->Lam;
                 a.nrow = nrow;
                 a.lx = D->lam[k].lx;
@@ -343,8 +416,7 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
                 a.lB = D->lB;
                 a.t = d->t;
                 a.ovf_list = d->ovf_list;
-                a.ovf_cnt = D->ovf_cnt;
-                a.global_blocks = ZPIC_GLOBAL_BLOCKS;
+                a.splits = D->splits;
                 /* the push constants of spec_advance */
                 a.tem = 0.5 * s->dt / s->m_q;
                 a.dt_dx = s->dt / s->dx[0];
@@ -382,12 +454,7 @@ void zpic_diff_delete(zpic_diff *D)
             zpic_dfree(CS(D, c, k)->tile_off);
         }
     }
-    for (int n = 0; n < D->ckpt; n++)
-        for (int k = 0; k < D->ns; k++) parts_free(&D->rp[(size_t)n * D->ns + k]);
-    for (int n = 0; n <= D->ckpt; n++) {
-        zpic_dfree(D->rE[n]);
-        zpic_dfree(D->rB[n]);
-    }
+    records_free(D);
     for (int k = 0; k < D->ns; k++) {
         vec_free(&D->lam[k]);
         vec_free(&D->lam2[k]);
@@ -400,14 +467,10 @@ void zpic_diff_delete(zpic_diff *D)
     zpic_dfree(D->aB);
     zpic_dfree(D->Lam);
     zpic_dfree(D->region);
-    zpic_dfree(D->ovf_cnt);
     zpic_dfree(D->out);
     free(D->cs);
     free(D->cE);
     free(D->cB);
-    free(D->rp);
-    free(D->rE);
-    free(D->rB);
     free(D->lam);
     free(D->lam2);
     free(D->pos);

@@ -21,8 +21,11 @@
  * the tile layout of the forward push: each tile stages its E,B window and
  * its window of the current adjoint in LDS, accumulates the adjoint of the
  * gathered fields in LDS windows, and stores them per tile; a node-parallel
- * pass adds the windows into the field adjoints.  Particles outside their
- * tile window are handled by a global pass with atomics.
+ * pass adds the windows into the field adjoints.  A tile's particles may be
+ * split over several workgroups (each with its own windows) so that small
+ * grids still fill the GPU.  Particles outside their tile window are
+ * listed and reversed by the same workgroup after the others, through the
+ * global grids with atomics.
  */
 #include "common.h"
 #include "zk_adjoint.h"
@@ -602,7 +605,9 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
 {
     extern __shared__ float sm[];
     const zk_tiles &t = a.t;
-    const int tile = blockIdx.x;
+    /* workgroup `part` of the tile's `splits` takes one contiguous share of
+     * the tile's particles and owns window slot blockIdx.x */
+    const int tile = blockIdx.x / a.splits, part = blockIdx.x % a.splits;
     const AdjConsts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
     const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
     const int n = t.WX * t.WY;
@@ -623,7 +628,9 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
     }
     __syncthreads();
     const LdsSrc src = {fl, ll, acc, t.WX, n};
-    const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
+    const int r0 = a.tile_off[tile], r1 = a.tile_off[tile + 1];
+    const int share = (r1 - r0 + a.splits - 1) / a.splits;
+    const int i0 = min(r1, r0 + part * share), i1 = min(r1, i0 + share);
     const int lane = __lane_id();
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const int ix = a.p.ix[i], iy = a.p.iy[i];
@@ -644,31 +651,27 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
         store_adj(a, i, l);
     }
     __syncthreads();
-    float *dst = a.region + (size_t)tile * 6 * n;
+    float *dst = a.region + (size_t)blockIdx.x * 6 * n;
     for (int m = threadIdx.x; m < 6 * n; m += blockDim.x) dst[m] = acc[m];
-    if (threadIdx.x == 0) a.ovf_cnt[tile] = nlist;
-}
-
-__global__ void __launch_bounds__(256) adj_global_kernel(zk_adj_push_args a)
-{
-    const AdjConsts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
-    const GlobalSrc src = {a.E, a.B, a.lam, a.lE, a.lB, a.nrow};
-    for (int tile = blockIdx.x; tile < a.t.ntiles; tile += gridDim.x) {
-        const int base = a.tile_off[tile], cnt = a.ovf_cnt[tile];
-        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-            const int i = a.ovf_list[base + q];
-            float l[5];
-            load_adj(a, i, l);
-            adjoint_one(src, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l,
-                        k);
-            store_adj(a, i, l);
-        }
+    /* the listed particles, through the guard-inclusive global grids: their
+     * field adjoints go straight into lE, lB with atomics (the windows are
+     * added by the next kernel) */
+    const int nl = nlist;
+    if (nl == 0) return;
+    const GlobalSrc gsrc = {a.E, a.B, a.lam, a.lE, a.lB, a.nrow};
+    for (int q = threadIdx.x; q < nl; q += blockDim.x) {
+        const int i = a.ovf_list[i0 + q];
+        float l[5];
+        load_adj(a, i, l);
+        adjoint_one(gsrc, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
+        store_adj(a, i, l);
     }
 }
 
-/* field adjoints += the tile windows covering each buffer node */
+/* field adjoints += the tile windows (splits slots per tile) covering each
+ * buffer node */
 __global__ void adj_reduce_kernel(const float *__restrict__ region, float *__restrict__ lE, float *__restrict__ lB,
-                                  zk_tiles t, int nrow)
+                                  zk_tiles t, int nrow, int splits)
 {
     const int bw = t.nx + 3, bh = t.ny + 3;
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
@@ -685,8 +688,9 @@ __global__ void adj_reduce_kernel(const float *__restrict__ region, float *__res
         for (int tx = txlo; tx <= txhi; tx++) {
             const int u = gx - (tx * t.TX - 1 - t.H), v = gy - (ty * t.TY - 1 - t.H);
             if (u < 0 || u >= t.WX || v < 0 || v >= t.WY) continue;
-            const float *w = region + (size_t)(tx + ty * t.ntx) * 6 * n + u + v * t.WX;
-            for (int c = 0; c < 6; c++) s[c] += w[c * n];
+            const float *w = region + (size_t)(tx + ty * t.ntx) * splits * 6 * n + u + v * t.WX;
+            for (int p = 0; p < splits; p++, w += 6 * n)
+                for (int c = 0; c < 6; c++) s[c] += w[c * n];
         }
     const size_t d = 3 * ((size_t)(gx + 1) + (size_t)(gy + 1) * nrow);
     for (int c = 0; c < 3; c++) {
@@ -812,11 +816,12 @@ extern "C" int zk_adj_push(const zk_adj_push_args *a, zk_stream_t s)
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e) return e;
     }
-    if (a->t.ntiles > 0) hipLaunchKernelGGL(adj_tile_kernel, dim3(a->t.ntiles), dim3(256), lds, s, *a);
-    hipLaunchKernelGGL(adj_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
+    if (a->splits < 1) return (int)hipErrorInvalidValue;
+    if (a->t.ntiles > 0)
+        hipLaunchKernelGGL(adj_tile_kernel, dim3(a->t.ntiles * a->splits), dim3(256), lds, s, *a);
     const int nodes = (a->t.nx + 3) * (a->t.ny + 3);
     hipLaunchKernelGGL(adj_reduce_kernel, dim3(grid_of((size_t)nodes)), dim3(256), 0, s, a->region, a->lE, a->lB,
-                       a->t, a->nrow);
+                       a->t, a->nrow, a->splits);
     ZK_LAUNCH_CHECK();
 }
 

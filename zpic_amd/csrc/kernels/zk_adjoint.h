@@ -27,12 +27,12 @@ typedef struct zk_adj_push_args {
     const float *lam;           /* adjoint of the raw current, per buffer node */
     int nrow;
     float *lx, *ly, *lux, *luy, *luz; /* in: after the step; out: before it */
-    float *region;              /* [ntiles][6][WY][WX] field-adjoint windows */
+    float *region;              /* [ntiles * splits][6][WY][WX] field-adjoint windows */
     float *lE, *lB;             /* field adjoints (accumulated) */
     zk_tiles t;
-    int *ovf_list;              /* slots outside their tile window, tile t's at tile_off[t] */
-    int *ovf_cnt;               /* [ntiles] listed per tile */
-    int global_blocks;
+    int *ovf_list;              /* slots outside their tile window, listed at the start
+                                   of the listing workgroup's share of the tile range */
+    int splits;                 /* workgroups per tile (>= 1) */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_adj_push_args;
 

@@ -50,6 +50,9 @@ class WeibelConfig:
     loss_last: int = 200
     block: int = 4
     seed: tuple = (12345, 67890)
+    # keep the state of every step during the forward run (memory of the
+    # whole trajectory) instead of running each segment again backwards
+    record: bool = False
 
 
 def _declare(lib):
@@ -64,6 +67,8 @@ def _declare(lib):
     lib.zpic_diff_backward.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), P(P(C.c_float))]
     lib.zpic_diff_delete.restype = None
     lib.zpic_diff_delete.argtypes = [C.c_void_p]
+    lib.zpic_diff_record_all.restype = None
+    lib.zpic_diff_record_all.argtypes = [C.c_void_p, C.c_int]
     lib._diff_declared = True
 
 
@@ -107,6 +112,8 @@ class WeibelRun:
         self.q = [float(self.species[k].q) for k in range(2)]
         self.initial = [A.particles_view(self.species[k]).copy() for k in range(2)]
         self.d = lib.zpic_diff_new(C.byref(self.sim), cfg.steps, cfg.ckpt, cfg.loss_first, cfg.loss_last)
+        if cfg.record:
+            lib.zpic_diff_record_all(self.d, 1)
 
     def block_index(self):
         cfg = self.cfg
@@ -136,7 +143,10 @@ class WeibelRun:
 
     def gradients(self):
         """(dL/duz, dL/duth, dL/dseed) from the adjoint of the initial state."""
-        dE, dB, parts = self.backward()
+        return self.reduce(*self.backward())
+
+    def reduce(self, dE, dB, parts):
+        """(dL/duz, dL/duth, dL/dseed) from the output of backward()."""
         g_uz = sum(self.sign[k] * float(parts[k][:, 4].astype(np.float64).sum()) for k in range(2))
         g_uth = sum(float((parts[k][:, 2:5].astype(np.float64) * self.G[k]).sum()) for k in range(2))
         cfg = self.cfg
