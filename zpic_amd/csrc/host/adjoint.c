@@ -53,6 +53,7 @@ struct zpic_diff {
     int **pos;
     float *lE, *lB, *lJ, *aE, *aB, *Lam, *region;
     int splits;        /* workgroups per tile in the reverse push */
+    unsigned *amax;    /* [ns] running maxima of the gathered-field adjoints (float bits) */
     float *out;        /* device staging of one species' adjoints by id */
     int forward_done;
     int iter0;
@@ -227,6 +228,7 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
     D->splits = t->ntiles > 0 ? (ZPIC_ADJ_BLOCKS + t->ntiles - 1) / t->ntiles : 1;
     if (D->splits > 8) D->splits = 8;
     D->region = zpic_dmalloc(sizeof(float) * 6 * (size_t)t->WX * t->WY * t->ntiles * D->splits);
+    D->amax = zpic_dmalloc(sizeof(unsigned) * (size_t)(D->ns > 0 ? D->ns : 1));
     D->out = zpic_dmalloc(sizeof(float) * 5 * (size_t)npmax);
     return D;
 }
@@ -366,6 +368,7 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
     ZCHECK(zk_fill_f32(D->lE, 3 * D->nodes, 0.0f, st));
     ZCHECK(zk_fill_f32(D->lB, 3 * D->nodes, 0.0f, st));
     for (int k = 0; k < D->ns; k++) vec_zero(&D->lam[k], D->np[k]);
+    ZCHECK(zk_fill_f32((float *)D->amax, (size_t)(D->ns > 0 ? D->ns : 1), 0.0f, st));
 
     for (int c = D->nck - 1; c >= 0; c--) {
         const int s0 = c * D->ckpt, s1 = (s0 + D->ckpt < D->nsteps) ? s0 + D->ckpt : D->nsteps;
@@ -406,11 +409,9 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
                 a.B = D->rB[r];
                 a.lam = D->Lam;
                 a.nrow = nrow;
-                a.lx = D->lam[k].lx;
-                a.ly = D->lam[k].ly;
-                a.lux = D->lam[k].lux;
-                a.luy = D->lam[k].luy;
-                a.luz = D->lam[k].luz;
+                a.lin = D->lam[k];
+                a.lout = D->lam2[k];
+                a.amax = D->amax + k;
                 a.region = D->region;
                 a.lE = D->lE;
                 a.lB = D->lB;
@@ -425,6 +426,9 @@ void zpic_diff_backward(zpic_diff *D, float *dE, float *dB, float *const *dpart)
                 a.qny = s->q * s->dx[1] / s->dt;
                 a.q = s->q;
                 ZCHECK(zk_adj_push(&a, st));
+                const zk_adj_vec tmp = D->lam[k];
+                D->lam[k] = D->lam2[k];
+                D->lam2[k] = tmp;
             }
         }
     }
@@ -467,6 +471,7 @@ void zpic_diff_delete(zpic_diff *D)
     zpic_dfree(D->aB);
     zpic_dfree(D->Lam);
     zpic_dfree(D->region);
+    zpic_dfree(D->amax);
     zpic_dfree(D->out);
     free(D->cs);
     free(D->cE);

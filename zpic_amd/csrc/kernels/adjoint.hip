@@ -21,7 +21,12 @@
  * the tile layout of the forward push: each tile stages its E,B window and
  * its window of the current adjoint in LDS, accumulates the adjoint of the
  * gathered fields in LDS windows, and stores them per tile; a node-parallel
- * pass adds the windows into the field adjoints.  A tile's particles may be
+ * pass adds the windows into the field adjoints.  The windows are summed in
+ * 64-bit fixed point (integer LDS atomics; a float LDS atomic costs about
+ * three integer ones here) with a power-of-two scale set from the running
+ * maximum of the summed quantities: 2^8 above it still fits, and a
+ * workgroup that meets a larger one sums its share again in float.  A
+ * tile's particles may be
  * split over several workgroups (each with its own windows) so that small
  * grids still fill the GPU.  Particles outside their tile window are
  * listed and reversed by the same workgroup after the others, through the
@@ -232,6 +237,18 @@ struct LdsSrc {
     __device__ float fld(int c, int a, int b) const { return f[c * n + a + b * w]; }
     __device__ float lam(int c, int a, int b) const { return l[c * n + a + b * w]; }
     __device__ void add(int c, int a, int b, float v) const { atomicAdd(&acc[c * n + a + b * w], v); }
+};
+
+/* the same windows with the gathered-field adjoints in fixed point */
+struct LdsSrcFx {
+    const float *f;
+    const float *l;
+    unsigned long long *acc;
+    int w, n;
+    float scale;
+    __device__ float fld(int c, int a, int b) const { return f[c * n + a + b * w]; }
+    __device__ float lam(int c, int a, int b) const { return l[c * n + a + b * w]; }
+    __device__ void add(int c, int a, int b, float v) const { atomicAdd(&acc[c * n + a + b * w], zk_fx(v, scale)); }
 };
 
 struct GlobalSrc {
@@ -485,10 +502,12 @@ struct AdjConsts {
 /* Reverse step of one particle: state (x, y, ux, uy, uz) at the start of
  * the step in cell (ci, cj) of the source's index space; on entry l[5]
  * holds the adjoints of (x, y, ux, uy, uz) after the step, on exit those
- * before it.  Field adjoints are added through src.add. */
+ * before it.  Field adjoints are added through src.add; the largest
+ * magnitude of the six gathered-field adjoints (the largest contribution
+ * to any window node) is returned. */
 template <class S>
-__device__ inline void adjoint_one(const S &src, int ci, int cj, float x, float y, float ux, float uy, float uz,
-                                   float l[5], const AdjConsts &k)
+__device__ inline float adjoint_one(const S &src, int ci, int cj, float x, float y, float ux, float uy, float uz,
+                                    float l[5], const AdjConsts &k)
 {
     const Stagger st(ci, cj, x, y);
     float f[6];
@@ -563,6 +582,8 @@ __device__ inline void adjoint_one(const S &src, int ci, int cj, float x, float 
     le1 += lay;
     le2 += laz;
     const float lf[6] = {le0 * k.tem, le1 * k.tem, le2 * k.tem, lb0, lb1, lb2};
+    const float lmax = fmaxf(fmaxf(fmaxf(fabsf(lf[0]), fabsf(lf[1])), fmaxf(fabsf(lf[2]), fabsf(lf[3]))),
+                             fmaxf(fabsf(lf[4]), fabsf(lf[5])));
     float gxs = 0.0f, gys = 0.0f;
     /* the derivatives of the gather are formed again here, which keeps
      * them out of the registers live across the deposit adjoint */
@@ -581,29 +602,73 @@ __device__ inline void adjoint_one(const S &src, int ci, int cj, float x, float 
     l[2] = lax;
     l[3] = lay;
     l[4] = laz;
+    /* NaN when any is NaN, so that a fixed-point pass rejects it */
+    return (lf[0] == lf[0] && lf[1] == lf[1] && lf[2] == lf[2] && lf[3] == lf[3] && lf[4] == lf[4] && lf[5] == lf[5])
+               ? lmax
+               : __builtin_nanf("");
 }
 
 __device__ inline void load_adj(const zk_adj_push_args &a, int i, float l[5])
 {
-    l[0] = a.lx[i];
-    l[1] = a.ly[i];
-    l[2] = a.lux[i];
-    l[3] = a.luy[i];
-    l[4] = a.luz[i];
+    l[0] = a.lin.lx[i];
+    l[1] = a.lin.ly[i];
+    l[2] = a.lin.lux[i];
+    l[3] = a.lin.luy[i];
+    l[4] = a.lin.luz[i];
 }
 
 __device__ inline void store_adj(const zk_adj_push_args &a, int i, const float l[5])
 {
-    a.lx[i] = l[0];
-    a.ly[i] = l[1];
-    a.lux[i] = l[2];
-    a.luy[i] = l[3];
-    a.luz[i] = l[4];
+    a.lout.lx[i] = l[0];
+    a.lout.ly[i] = l[1];
+    a.lout.lux[i] = l[2];
+    a.lout.luy[i] = l[3];
+    a.lout.luz[i] = l[4];
+}
+
+/* contributions up to 2^46 per atomic: sums of up to 2^17 of them fit */
+#define ZK_ADJ_FX_LIMIT 70368744177664.0f /* 2^46 */
+#define ZK_ADJ_FX_MARGIN 8                /* 2^8 above the running maximum */
+#define ZK_ADJ_FX_MAX_SHARE 131072
+
+/* The windowed particles of [i0, i1): particles outside the window are
+ * listed (at ovf_list[i0...]) instead.  Returns the largest gathered-field
+ * adjoint met (NaN if a NaN was met). */
+template <class S>
+__device__ inline float adj_share(const zk_adj_push_args &a, const S &src, int i0, int i1, int tx0, int ty0, int ox,
+                                  int oy, const AdjConsts &k, int *nlist)
+{
+    const zk_tiles &t = a.t;
+    const int lane = __lane_id();
+    float mx = 0.0f;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int ix = a.p.ix[i], iy = a.p.iy[i];
+        const int lx = ix - tx0, ly = iy - ty0;
+        const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
+        const unsigned long long out = __ballot(!inside);
+        if (out) {
+            int base = 0;
+            const int leader = __ffsll((long long)out) - 1;
+            if (lane == leader) base = atomicAdd(nlist, __popcll(out));
+            base = __shfl(base, leader, ZK_WAVE);
+            if (!inside) a.ovf_list[i0 + base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+        }
+        if (!inside) continue;
+        float l[5];
+        load_adj(a, i, l);
+        const float m = adjoint_one(src, ix - ox, iy - oy, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
+        mx = (m == m && mx == mx) ? fmaxf(mx, m) : __builtin_nanf("");
+        store_adj(a, i, l);
+    }
+    return mx;
 }
 
 __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
 {
-    extern __shared__ float sm[];
+    /* declared as 64-bit words so that the dynamic LDS starts 8-byte
+     * aligned after the static variables (ds_add_u64 needs it) */
+    extern __shared__ unsigned long long smx[];
+    float *sm = (float *)smx;
     const zk_tiles &t = a.t;
     /* workgroup `part` of the tile's `splits` takes one contiguous share of
      * the tile's particles and owns window slot blockIdx.x */
@@ -612,9 +677,28 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
     const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
     const int n = t.WX * t.WY;
     const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H;
-    float *fl = sm, *ll = sm + 6 * n, *acc = sm + 9 * n;
-    __shared__ int nlist;
-    if (threadIdx.x == 0) nlist = 0;
+    float *fl = sm, *ll = sm + 6 * n;
+    unsigned long long *accx = smx + (9 * n + 1) / 2; /* after the 9 n floats */
+    float *accf = (float *)accx;
+    __shared__ int nlist, bad;
+    __shared__ unsigned bmax;
+    const int r0 = a.tile_off[tile], r1 = a.tile_off[tile + 1];
+    const int share = (r1 - r0 + a.splits - 1) / a.splits;
+    const int i0 = min(r1, r0 + part * share), i1 = min(r1, i0 + share);
+    /* the scale: a power of two with 2^8 times the running maximum at
+     * 2^46; none (float sums) before any maximum is known or when the
+     * scale would not be a normal float */
+    const float amax = __uint_as_float(__atomic_load_n(a.amax, __ATOMIC_RELAXED));
+    int e = 0;
+    frexpf(amax, &e); /* amax < 2^e */
+    const int se = 46 - ZK_ADJ_FX_MARGIN - e;
+    const bool fx = amax > 0.0f && amax <= 3.0e38f && se > -120 && se < 120 && i1 - i0 <= ZK_ADJ_FX_MAX_SHARE;
+    const float scale = fx ? ldexpf(1.0f, se) : 0.0f;
+    if (threadIdx.x == 0) {
+        nlist = 0;
+        bad = 0;
+        bmax = 0u;
+    }
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
         const int gx = ox + m % t.WX, gy = oy + m / t.WX;
         const bool in = gx >= -1 && gx <= t.nx + 1 && gy >= -1 && gy <= t.ny + 1;
@@ -624,35 +708,39 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
             fl[(3 + c) * n + m] = in ? a.B[g + c] : 0.0f;
             ll[c * n + m] = in ? a.lam[g + c] : 0.0f;
         }
-        for (int c = 0; c < 6; c++) acc[c * n + m] = 0.0f;
     }
+    for (int m = threadIdx.x; m < 6 * n; m += blockDim.x) accx[m] = 0ull;
     __syncthreads();
-    const LdsSrc src = {fl, ll, acc, t.WX, n};
-    const int r0 = a.tile_off[tile], r1 = a.tile_off[tile + 1];
-    const int share = (r1 - r0 + a.splits - 1) / a.splits;
-    const int i0 = min(r1, r0 + part * share), i1 = min(r1, i0 + share);
-    const int lane = __lane_id();
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const int ix = a.p.ix[i], iy = a.p.iy[i];
-        const int lx = ix - tx0, ly = iy - ty0;
-        const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
-        const unsigned long long out = __ballot(!inside);
-        if (out) {
-            int base = 0;
-            const int leader = __ffsll((long long)out) - 1;
-            if (lane == leader) base = atomicAdd(&nlist, __popcll(out));
-            base = __shfl(base, leader, ZK_WAVE);
-            if (!inside) a.ovf_list[i0 + base + __popcll(out & ((1ull << lane) - 1ull))] = i;
+    float mx;
+    bool fixed = fx;
+    if (fx) {
+        const LdsSrcFx src = {fl, ll, accx, t.WX, n, scale};
+        mx = adj_share(a, src, i0, i1, tx0, ty0, ox, oy, k, &nlist);
+        if (!(mx * scale < ZK_ADJ_FX_LIMIT)) bad = 1;
+        __syncthreads();
+        if (bad) {
+            /* a contribution beyond the fixed-point range: this share again
+             * with float sums (the particle adjoints are read from lin and
+             * written to lout, so the pass can be repeated) */
+            fixed = false;
+            for (int m = threadIdx.x; m < 6 * n; m += blockDim.x) accx[m] = 0ull;
+            if (threadIdx.x == 0) nlist = 0;
+            __syncthreads();
         }
-        if (!inside) continue;
-        float l[5];
-        load_adj(a, i, l);
-        adjoint_one(src, ix - ox, iy - oy, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
-        store_adj(a, i, l);
     }
+    if (!fixed) {
+        const LdsSrc src = {fl, ll, accf, t.WX, n};
+        mx = adj_share(a, src, i0, i1, tx0, ty0, ox, oy, k, &nlist);
+    }
+    /* running maximum of this species: block maximum, one global atomic */
+    for (int o = ZK_WAVE / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, ZK_WAVE));
+    if (__lane_id() == 0 && mx > 0.0f) atomicMax(&bmax, __float_as_uint(mx));
     __syncthreads();
+    if (threadIdx.x == 0 && bmax) atomicMax(a.amax, bmax);
     float *dst = a.region + (size_t)blockIdx.x * 6 * n;
-    for (int m = threadIdx.x; m < 6 * n; m += blockDim.x) dst[m] = acc[m];
+    const float unscale = fixed ? 1.0f / scale : 0.0f;
+    for (int m = threadIdx.x; m < 6 * n; m += blockDim.x)
+        dst[m] = fixed ? (float)(long long)accx[m] * unscale : accf[m];
     /* the listed particles, through the guard-inclusive global grids: their
      * field adjoints go straight into lE, lB with atomics (the windows are
      * added by the next kernel) */
@@ -663,7 +751,9 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
         const int i = a.ovf_list[i0 + q];
         float l[5];
         load_adj(a, i, l);
-        adjoint_one(gsrc, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
+        const float m =
+            adjoint_one(gsrc, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
+        if (m > 0.0f) atomicMax(a.amax, __float_as_uint(m));
         store_adj(a, i, l);
     }
 }
@@ -810,7 +900,7 @@ extern "C" int zk_adj_images(const float *lJ, float *lam, int nx, int ny, int nr
 extern "C" int zk_adj_push(const zk_adj_push_args *a, zk_stream_t s)
 {
     const int n = a->t.WX * a->t.WY;
-    const size_t lds = (size_t)15 * n * sizeof(float);
+    const size_t lds = ((size_t)(9 * n + 1) / 2 + (size_t)6 * n) * sizeof(unsigned long long);
     if (lds > 65536) {
         const int e = (int)hipFuncSetAttribute((const void *)adj_tile_kernel,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -22,6 +22,19 @@ __device__ inline int zk_xcd_item(int b, int n)
     return x * base + (x < rem ? x : rem) + k;
 }
 
+/* Float to 64-bit fixed point: trunc(v * scale) for a power-of-two scale.
+ * x = v * scale is exact; hi = trunc(x / 2^31) and the remainder
+ * lo = x - 2^31 hi (exact in float, |lo| < 2^31) convert with one
+ * instruction each, and trunc(x) = 2^31 hi + trunc(lo).  Valid for
+ * |x| < 2^55. */
+__device__ inline unsigned long long zk_fx(float v, float scale)
+{
+    const float x = v * scale;
+    const float hi = truncf(x * 4.656612873077393e-10f); /* x / 2^31 */
+    const float lo = x - hi * 2147483648.0f;             /* exact */
+    return (unsigned long long)(((long long)(int)hi << 31) + (long long)(int)lo);
+}
+
 /* Floor modulo for periodic wrap. */
 __device__ inline int zk_wrap(int i, int n)
 {

@@ -50,20 +50,8 @@ struct LdsCurrent {
 
 /* Fixed-point current sink: contributions are divided by Q (the power of
  * two at or above the particle charge |q|; every contribution is O(|q|)),
- * scaled by 2^40 and truncated to 64-bit integers, which integer LDS
- * atomics sum exactly, independent of the order of the atomics.
- * zk_fx(v, scale), scale = 2^40 / Q: x = v * scale is exact (a power of
- * two); hi = trunc(x / 2^31) and the remainder lo = x - 2^31 hi (exact in
- * float, |lo| < 2^31) convert with one instruction each, and
- * trunc(x) = 2^31 hi + trunc(lo).  Valid for |x| < 2^55. */
-__device__ inline unsigned long long zk_fx(float v, float scale)
-{
-    const float x = v * scale;
-    const float hi = truncf(x * 4.656612873077393e-10f); /* x / 2^31 */
-    const float lo = x - hi * 2147483648.0f;             /* exact */
-    return (unsigned long long)(((long long)(int)hi << 31) + (long long)(int)lo);
-}
-
+ * scaled by 2^40 and truncated to 64-bit integers (zk_fx), which integer
+ * LDS atomics sum exactly, independent of the order of the atomics. */
 struct LdsCurrentFx {
     unsigned long long *j;
     int w, n;

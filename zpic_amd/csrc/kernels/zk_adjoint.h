@@ -26,13 +26,16 @@ typedef struct zk_adj_push_args {
     const float *E, *B;         /* fields at the start of the step */
     const float *lam;           /* adjoint of the raw current, per buffer node */
     int nrow;
-    float *lx, *ly, *lux, *luy, *luz; /* in: after the step; out: before it */
+    zk_adj_vec lin;             /* particle adjoints after the step (read) */
+    zk_adj_vec lout;            /* particle adjoints before the step (written) */
     float *region;              /* [ntiles * splits][6][WY][WX] field-adjoint windows */
     float *lE, *lB;             /* field adjoints (accumulated) */
     zk_tiles t;
     int *ovf_list;              /* slots outside their tile window, listed at the start
                                    of the listing workgroup's share of the tile range */
     int splits;                 /* workgroups per tile (>= 1) */
+    unsigned *amax;             /* float bits: running maximum of the gathered-field
+                                   adjoints of this species (0 at the start) */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_adj_push_args;
 
