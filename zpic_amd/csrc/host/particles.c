@@ -292,7 +292,9 @@ static void species_free_device(struct zpic_dev_species *d)
     zpic_dfree(d->tile_off);
     zpic_dfree(d->tile_off2);
     zpic_dfree(d->stay);
-    d->tile_off2 = d->stay = NULL;
+    zpic_dfree(d->nbr);
+    zpic_dfree(d->tailn);
+    d->tile_off2 = d->stay = d->nbr = d->tailn = NULL;
     zpic_dfree(d->cnt);
     zpic_dfree(d->cursor);
     zpic_dfree(d->epart);
@@ -423,8 +425,8 @@ static void sort_now(t_species *s, int spread)
                        spread, zpic_stream()));
     } else {
         /* the particles are in tile ranges (plus a tail): keep the stayers */
-        ZCHECK(zk_sort_tiles(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->stay, d->tile_off, d->tile_off2,
-                             d->cursor, &d->t, d->tail_upper, zpic_stream()));
+        ZCHECK(zk_sort_tiles(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->stay, d->nbr, d->tailn, d->tile_off,
+                             d->tile_off2, d->cursor, &d->t, d->tail_upper, zpic_stream()));
         int *tmp = d->tile_off;
         d->tile_off = d->tile_off2;
         d->tile_off2 = tmp;
@@ -488,6 +490,9 @@ void zpic_species_ensure(t_species *s)
         d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->tile_off2 = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->stay = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->nbr = zpic_dmalloc(sizeof(int) * 9 * (size_t)d->t.ntiles);
+        d->tailn = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        ZCHECK(zk_fill_f32((float *)d->tailn, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */
         d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         d->cursor = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         ZCHECK(zk_fill_f32((float *)d->cnt, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */

@@ -38,6 +38,7 @@ struct zpic_dev_species {
     zk_tiles t;
     int *tile_off, *cnt, *cursor;
     int *tile_off2, *stay; /* second range array and stayer counts of the re-sort */
+    int *nbr, *tailn;      /* re-sort: mover runs per neighbour slot, farther movers per tile */
     int *ovf_list;        /* slots listed by the tile pass for the global pass */
     double *epart;
     int epart_cap;

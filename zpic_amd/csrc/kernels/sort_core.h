@@ -110,38 +110,77 @@ __device__ inline void scatter_body(const zk_parts &src, const zk_parts &dst, co
 
 /* ---- sort of a tile-sorted input: particles that stay in their tile keep
  * their order at the front of the tile's new range, the others are
- * appended after the stayers of their new tile ---- */
+ * appended after the stayers of their new tile ----
+ *
+ * Particles move at most a few cells between sorts, so nearly every mover
+ * lands in one of the 8 tiles around its source tile.  The count pass
+ * counts, per source tile, its stayers and its movers per neighbour slot
+ * (LDS counters, one ballot per distinct slot in a wave) and reserves each
+ * slot's run in the destination's mover region with one global atomic
+ * (nbr[9 b + slot] = start of the run).  The scatter pass writes stayers
+ * in order (each wave owns a contiguous piece of the tile, its offset from
+ * a per-wave count) and movers at their run start plus an LDS rank.  The
+ * rare particle that moved farther, and the unsorted tail, reserve slots
+ * after the movers with wave-aggregated global atomics (tailn, cursor). */
 
-/* Exclusive prefix of `flag` over the block in thread order; returns the
- * rank and sets *total.  lds needs blockDim.x / 64 ints. */
-__device__ inline int block_rank(bool flag, int *lds, int *total)
+/* neighbour slot 0..8 of tile `key` seen from tile `b` (periodic), 4 for
+ * b itself, -1 when not adjacent */
+__device__ inline int nbr_slot(int key, int b, const zk_tiles &t)
 {
-    const int lane = __lane_id(), w = threadIdx.x / ZK_WAVE, nw = blockDim.x / ZK_WAVE;
-    const unsigned long long m = __ballot(flag);
-    const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) lds[w] = __popcll(m);
-    __syncthreads();
-    int before = 0, all = 0;
-    for (int k = 0; k < nw; k++) {
-        const int c = lds[k];
-        before += (k < w) ? c : 0;
-        all += c;
-    }
-    __syncthreads();
-    *total = all;
-    return before + in_wave;
+    int dx = key % t.ntx - b % t.ntx, dy = key / t.ntx - b / t.ntx;
+    if (dx > 1) dx -= t.ntx;
+    else if (dx < -1) dx += t.ntx;
+    if (dy > 1) dy -= t.nty;
+    else if (dy < -1) dy += t.nty;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1) return -1;
+    return (dx + 1) + 3 * (dy + 1);
 }
 
-/* Block b < ntiles counts the stayers of source tile b (stay[b]) and
- * reserves the movers in move[]; blocks beyond take chunks of the tail.
- * old_off[ntiles] is the end of the tile ranges, old_off[ntiles + 1] the
- * slot count in use. */
+/* tile in neighbour slot j of tile b */
+__device__ inline int nbr_tile(int j, int b, const zk_tiles &t)
+{
+    const int tx = (b % t.ntx + j % 3 - 1 + t.ntx) % t.ntx;
+    const int ty = (b / t.ntx + j / 3 - 1 + t.nty) % t.nty;
+    return tx + ty * t.ntx;
+}
+
+/* Adds each lane with slot >= 0 to lcnt[slot] (LDS), one atomic per
+ * distinct slot in the wave; returns the lane's rank: the counter before
+ * the wave's addition plus the lanes of the same slot below it.  Must be
+ * called by all 64 lanes of the wave. */
+__device__ inline int wave_slot_add(int slot, int *lcnt)
+{
+    const int lane = __lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(slot >= 0);
+    int mine = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int sl = __shfl(slot, leader, ZK_WAVE);
+        const unsigned long long m = __ballot(slot == sl);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&lcnt[sl], __popcll(m));
+        base = __shfl(base, leader, ZK_WAVE);
+        if (slot == sl) mine = base + __popcll(m & lt);
+        todo &= ~m;
+    }
+    return mine;
+}
+
+/* Block b < ntiles counts the stayers of source tile b (stay[b]) and its
+ * movers per neighbour slot, reserving their runs (nbr, move); farther
+ * movers count in tailn.  Blocks beyond count chunks of the tail into
+ * tailn.  old_off[ntiles] is the end of the tile ranges, old_off[ntiles +
+ * 1] the slot count in use. */
 __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__restrict__ iy,
-                                       const int *__restrict__ old_off, int *stay, int *move, const zk_tiles &t,
-                                       const int block, const int nblocks)
+                                       const int *__restrict__ old_off, int *stay, int *move, int *nbr, int *tailn,
+                                       const zk_tiles &t, const int block, const int nblocks)
 {
     __shared__ int red[16];
+    __shared__ int lc[9];
     if (block < t.ntiles) {
+        if (threadIdx.x < 9) lc[threadIdx.x] = 0;
+        __syncthreads();
         const int lo = old_off[block], hi = old_off[block + 1];
         int nstay = 0;
         for (int base = lo; base < hi; base += blockDim.x) {
@@ -149,7 +188,9 @@ __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__
             const int key = (i < hi) ? tile_key(ix[i], iy[i], t) : -1;
             const bool st = key == block;
             nstay += st;
-            zk_run_reserve(st ? -1 : key, move);
+            const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
+            wave_slot_add(slot, lc);
+            zk_run_reserve((key >= 0 && !st && slot < 0) ? key : -1, tailn);
         }
         for (int o = ZK_WAVE / 2; o > 0; o >>= 1) nstay += __shfl_down(nstay, o, ZK_WAVE);
         if ((threadIdx.x & (ZK_WAVE - 1)) == 0) red[threadIdx.x / ZK_WAVE] = nstay;
@@ -159,27 +200,32 @@ __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__
             for (int k = 0; k < (int)(blockDim.x / ZK_WAVE); k++) s += red[k];
             stay[block] = s;
         }
+        if (threadIdx.x < 9) {
+            const int c = lc[threadIdx.x];
+            nbr[9 * block + threadIdx.x] = c ? atomicAdd(&move[nbr_tile(threadIdx.x, block, t)], c) : 0;
+        }
     } else {
         const int n0 = old_off[t.ntiles], n1 = old_off[t.ntiles + 1];
         const int stride = (nblocks - t.ntiles) * blockDim.x;
         for (int base = n0 + (block - t.ntiles) * blockDim.x; base < n1; base += stride) {
             const int i = base + threadIdx.x;
             const int key = (i < n1) ? tile_key(ix[i], iy[i], t) : -1;
-            zk_run_reserve(key, move);
+            zk_run_reserve(key, tailn);
         }
     }
 }
 
-/* new_off = exclusive scan of stay + move; cursor = new_off + stay (where
- * the movers go); move is cleared.  One workgroup of 1024 threads. */
-__device__ inline void scan_stay_body(const int *stay, int *move, int *new_off, int *cursor, zk_spec_ctr *ctr,
-                                      int ntiles)
+/* new_off = exclusive scan of stay + move + tailn; cursor = new_off +
+ * stay + move (where the farther movers and the tail go); move and tailn
+ * are cleared.  One workgroup of 1024 threads. */
+__device__ inline void scan_stay_body(const int *stay, int *move, int *tailn, int *new_off, int *cursor,
+                                      zk_spec_ctr *ctr, int ntiles)
 {
     __shared__ int part[1024];
     const int per = (ntiles + blockDim.x - 1) / blockDim.x;
     const int b = threadIdx.x * per, e = min(ntiles, b + per);
     int s = 0;
-    for (int i = b; i < e; i++) s += stay[i] + move[i];
+    for (int i = b; i < e; i++) s += stay[i] + move[i] + tailn[i];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int o = 1; o < (int)blockDim.x; o <<= 1) {
@@ -190,10 +236,11 @@ __device__ inline void scan_stay_body(const int *stay, int *move, int *new_off, 
     }
     int run = part[threadIdx.x] - s;
     for (int i = b; i < e; i++) {
-        const int c = stay[i] + move[i];
+        const int c = stay[i] + move[i] + tailn[i];
         new_off[i] = run;
-        cursor[i] = run + stay[i];
+        cursor[i] = run + stay[i] + move[i];
         move[i] = 0;
+        tailn[i] = 0;
         run += c;
     }
     if (threadIdx.x == blockDim.x - 1) {
@@ -217,32 +264,53 @@ __device__ inline void copy_part(const zk_parts &src, const zk_parts &dst, int i
 }
 
 /* Block b < ntiles moves source tile b: stayers to new_off[b] + their
- * rank among the tile's stayers (contiguous, in order), movers to the
- * cursor of their new tile; blocks beyond move chunks of the tail. */
+ * rank among the tile's stayers (contiguous, in order), movers to their
+ * slot's run; blocks beyond move chunks of the tail. */
 __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &dst, const int *__restrict__ old_off,
-                                         const int *__restrict__ new_off, int *cursor, const zk_tiles &t,
+                                         const int *__restrict__ new_off, const int *__restrict__ stay,
+                                         const int *__restrict__ nbr, int *cursor, const zk_tiles &t,
                                          const int block, const int nblocks)
 {
-    __shared__ int red[16];
+    __shared__ int lc[9], run0[9], wst[16];
+    const int lane = __lane_id();
     if (block < t.ntiles) {
+        if (threadIdx.x < 9) {
+            const int d = nbr_tile(threadIdx.x, block, t);
+            lc[threadIdx.x] = 0;
+            run0[threadIdx.x] = new_off[d] + stay[d] + nbr[9 * block + threadIdx.x];
+        }
+        /* wave w owns the contiguous piece [wlo, whi) of the tile */
         const int lo = old_off[block], hi = old_off[block + 1];
-        const int dst0 = new_off[block];
-        int placed = 0;
-        for (int base = lo; base < hi; base += blockDim.x) {
-            const int i = base + threadIdx.x;
+        const int nw = blockDim.x / ZK_WAVE, w = threadIdx.x / ZK_WAVE;
+        const int piece = (hi - lo + nw - 1) / nw;
+        const int wlo = min(hi, lo + w * piece), whi = min(hi, wlo + piece);
+        int nst = 0;
+        for (int i = wlo + lane; i < whi; i += ZK_WAVE) nst += tile_key(src.ix[i], src.iy[i], t) == block;
+        for (int o = ZK_WAVE / 2; o > 0; o >>= 1) nst += __shfl_down(nst, o, ZK_WAVE);
+        if (lane == 0) wst[w] = nst;
+        __syncthreads();
+        int placed = new_off[block];
+        for (int k = 0; k < w; k++) placed += wst[k];
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        for (int base = wlo; base < whi; base += ZK_WAVE) {
+            const int i = base + lane;
             int ix = 0, iy = 0, key = -1;
-            if (i < hi) {
+            if (i < whi) {
                 ix = src.ix[i];
                 iy = src.iy[i];
                 key = tile_key(ix, iy, t);
             }
             const bool st = key == block;
-            int total;
-            const int r = block_rank(st, red, &total);
-            const int dm = zk_run_reserve(st ? -1 : key, cursor);
-            if (st) copy_part(src, dst, i, dst0 + placed + r, ix, iy);
-            else if (key >= 0) copy_part(src, dst, i, dm, ix, iy);
-            placed += total;
+            const unsigned long long ms = __ballot(st);
+            int d = placed + __popcll(ms & lt);
+            placed += __popcll(ms);
+            const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
+            const int r = wave_slot_add(slot, lc);
+            if (slot >= 0) d = run0[slot] + r;
+            const bool far = key >= 0 && !st && slot < 0;
+            const int dm = zk_run_reserve(far ? key : -1, cursor);
+            if (far) d = dm;
+            if (key >= 0) copy_part(src, dst, i, d, ix, iy);
         }
     } else {
         const int n0 = old_off[t.ntiles], n1 = old_off[t.ntiles + 1];
