@@ -490,7 +490,7 @@ void zpic_species_ensure(t_species *s)
         d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->tile_off2 = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->stay = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
-        d->nbr = zpic_dmalloc(sizeof(int) * 9 * (size_t)d->t.ntiles);
+        d->nbr = zpic_dmalloc(sizeof(int) * 13 * (size_t)d->t.ntiles);
         d->tailn = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
         ZCHECK(zk_fill_f32((float *)d->tailn, (size_t)d->t.ntiles, 0.0f, zpic_stream())); /* int zeros */
         d->cnt = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);

@@ -49,8 +49,9 @@ __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, i
     scatter_body(src, dst, n_old, cursor, off, t, spread, blockIdx.x, gridDim.x);
 }
 
-__global__ void count_stay_kernel(const int *__restrict__ ix, const int *__restrict__ iy, const int *old_off,
-                                  int *stay, int *move, int *nbr, int *tailn, zk_tiles t)
+__global__ void __launch_bounds__(ZK_SORT_THREADS)
+    count_stay_kernel(const int *__restrict__ ix, const int *__restrict__ iy, const int *old_off, int *stay, int *move,
+                      int *nbr, int *tailn, zk_tiles t)
 {
     count_stay_body(ix, iy, old_off, stay, move, nbr, tailn, t, blockIdx.x, gridDim.x);
 }
@@ -61,8 +62,9 @@ __global__ void __launch_bounds__(1024) scan_stay_kernel(const int *stay, int *m
     scan_stay_body(stay, move, tailn, new_off, cursor, ctr, ntiles);
 }
 
-__global__ void scatter_stay_kernel(zk_parts src, zk_parts dst, const int *old_off, const int *new_off,
-                                    const int *stay, const int *nbr, int *cursor, zk_tiles t)
+__global__ void __launch_bounds__(ZK_SORT_THREADS)
+    scatter_stay_kernel(zk_parts src, zk_parts dst, const int *old_off, const int *new_off, const int *stay,
+                        const int *nbr, int *cursor, zk_tiles t)
 {
     scatter_stay_body(src, dst, old_off, new_off, stay, nbr, cursor, t, blockIdx.x, gridDim.x);
 }
@@ -146,12 +148,12 @@ extern "C" int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_c
     const unsigned tb = tail_upper > 0 ? grid_for(tail_upper, 1024, 4096) : 0;
     const unsigned g = (unsigned)t->ntiles + tb;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, old_off + t->ntiles + 1);
-    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, old_off, stay, move, nbr, tailn,
-                       *t);
+    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, src->ix, src->iy, old_off, stay, move,
+                       nbr, tailn, *t);
     hipLaunchKernelGGL(scan_stay_kernel, dim3(1), dim3(1024), 0, s, stay, move, tailn, new_off, cursor, ctr,
                        t->ntiles);
-    hipLaunchKernelGGL(scatter_stay_kernel, dim3(g), dim3(256), 0, s, *src, *dst, old_off, new_off, stay, nbr, cursor,
-                       *t);
+    hipLaunchKernelGGL(scatter_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, *src, *dst, old_off, new_off, stay,
+                       nbr, cursor, *t);
     ZK_LAUNCH_CHECK();
 }
 

@@ -117,11 +117,28 @@ __device__ inline void scatter_body(const zk_parts &src, const zk_parts &dst, co
  * counts, per source tile, its stayers and its movers per neighbour slot
  * (LDS counters, one ballot per distinct slot in a wave) and reserves each
  * slot's run in the destination's mover region with one global atomic
- * (nbr[9 b + slot] = start of the run).  The scatter pass writes stayers
- * in order (each wave owns a contiguous piece of the tile, its offset from
- * a per-wave count) and movers at their run start plus an LDS rank.  The
+ * (nbr[ZK_NBR b + slot] = start of the run).  Each of the block's waves
+ * owns a contiguous piece of the tile and records its stayer count
+ * (nbr[ZK_NBR b + 9 + wave]).  The scatter pass writes stayers in order,
+ * each wave from its offset, and movers at their run start plus an LDS
+ * rank.  The
  * rare particle that moved farther, and the unsorted tail, reserve slots
  * after the movers with wave-aggregated global atomics (tailn, cursor). */
+
+/* per tile in nbr: 9 run starts and the stayer counts of the 4 waves of a
+ * 256-thread block */
+#define ZK_NBR 13
+#define ZK_SORT_THREADS 256
+/* 64-particle chunks a wave loads before processing them (loads in flight) */
+#define ZK_SORT_UNROLL 4
+
+/* piece [*lo, *hi) of the tile range [a, b) owned by wave w of nw */
+__device__ inline void wave_piece(int a, int b, int w, int nw, int *lo, int *hi)
+{
+    const int piece = (b - a + nw - 1) / nw;
+    *lo = min(b, a + w * piece);
+    *hi = min(b, *lo + piece);
+}
 
 /* neighbour slot 0..8 of tile `key` seen from tile `b` (periodic), 4 for
  * b itself, -1 when not adjacent */
@@ -167,42 +184,55 @@ __device__ inline int wave_slot_add(int slot, int *lcnt)
     return mine;
 }
 
-/* Block b < ntiles counts the stayers of source tile b (stay[b]) and its
- * movers per neighbour slot, reserving their runs (nbr, move); farther
- * movers count in tailn.  Blocks beyond count chunks of the tail into
- * tailn.  old_off[ntiles] is the end of the tile ranges, old_off[ntiles +
- * 1] the slot count in use. */
+/* Block b < ntiles (ZK_SORT_THREADS threads) counts the stayers of source
+ * tile b (stay[b], and per wave in nbr) and its movers per neighbour slot,
+ * reserving their runs (nbr, move); farther movers count in tailn.  Blocks
+ * beyond count chunks of the tail into tailn.  old_off[ntiles] is the end
+ * of the tile ranges, old_off[ntiles + 1] the slot count in use. */
 __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__restrict__ iy,
                                        const int *__restrict__ old_off, int *stay, int *move, int *nbr, int *tailn,
                                        const zk_tiles &t, const int block, const int nblocks)
 {
-    __shared__ int red[16];
-    __shared__ int lc[9];
+    __shared__ int lc[9], wst[ZK_SORT_THREADS / ZK_WAVE];
+    const int lane = __lane_id();
     if (block < t.ntiles) {
         if (threadIdx.x < 9) lc[threadIdx.x] = 0;
         __syncthreads();
-        const int lo = old_off[block], hi = old_off[block + 1];
+        const int nw = ZK_SORT_THREADS / ZK_WAVE, w = threadIdx.x / ZK_WAVE;
+        int wlo, whi;
+        wave_piece(old_off[block], old_off[block + 1], w, nw, &wlo, &whi);
         int nstay = 0;
-        for (int base = lo; base < hi; base += blockDim.x) {
-            const int i = base + threadIdx.x;
-            const int key = (i < hi) ? tile_key(ix[i], iy[i], t) : -1;
-            const bool st = key == block;
-            nstay += st;
-            const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
-            wave_slot_add(slot, lc);
-            zk_run_reserve((key >= 0 && !st && slot < 0) ? key : -1, tailn);
+        for (int base = wlo; base < whi; base += ZK_SORT_UNROLL * ZK_WAVE) {
+            int cx[ZK_SORT_UNROLL], cy[ZK_SORT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < ZK_SORT_UNROLL; u++) {
+                const int i = base + u * ZK_WAVE + lane;
+                cx[u] = (i < whi) ? ix[i] : ZK_DEAD;
+                cy[u] = (i < whi) ? iy[i] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < ZK_SORT_UNROLL; u++) {
+                const int key = tile_key(cx[u], cy[u], t);
+                const bool st = key == block;
+                nstay += __popcll(__ballot(st));
+                const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
+                wave_slot_add(slot, lc);
+                zk_run_reserve((key >= 0 && !st && slot < 0) ? key : -1, tailn);
+            }
         }
-        for (int o = ZK_WAVE / 2; o > 0; o >>= 1) nstay += __shfl_down(nstay, o, ZK_WAVE);
-        if ((threadIdx.x & (ZK_WAVE - 1)) == 0) red[threadIdx.x / ZK_WAVE] = nstay;
+        if (lane == 0) {
+            wst[w] = nstay;
+            nbr[ZK_NBR * block + 9 + w] = nstay;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             int s = 0;
-            for (int k = 0; k < (int)(blockDim.x / ZK_WAVE); k++) s += red[k];
+            for (int k = 0; k < nw; k++) s += wst[k];
             stay[block] = s;
         }
         if (threadIdx.x < 9) {
             const int c = lc[threadIdx.x];
-            nbr[9 * block + threadIdx.x] = c ? atomicAdd(&move[nbr_tile(threadIdx.x, block, t)], c) : 0;
+            nbr[ZK_NBR * block + threadIdx.x] = c ? atomicAdd(&move[nbr_tile(threadIdx.x, block, t)], c) : 0;
         }
     } else {
         const int n0 = old_off[t.ntiles], n1 = old_off[t.ntiles + 1];
@@ -263,54 +293,72 @@ __device__ inline void copy_part(const zk_parts &src, const zk_parts &dst, int i
     if (src.id) dst.id[d] = src.id[i];
 }
 
-/* Block b < ntiles moves source tile b: stayers to new_off[b] + their
- * rank among the tile's stayers (contiguous, in order), movers to their
- * slot's run; blocks beyond move chunks of the tail. */
+/* Block b < ntiles (ZK_SORT_THREADS threads) moves source tile b:
+ * stayers to new_off[b] + their rank among the tile's stayers (contiguous,
+ * in order), movers to their slot's run; blocks beyond move chunks of the
+ * tail.  Every lane loads its particle before the slot bookkeeping, so the
+ * loads are in flight during it. */
 __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &dst, const int *__restrict__ old_off,
                                          const int *__restrict__ new_off, const int *__restrict__ stay,
                                          const int *__restrict__ nbr, int *cursor, const zk_tiles &t,
                                          const int block, const int nblocks)
 {
-    __shared__ int lc[9], run0[9], wst[16];
+    __shared__ int lc[9], run0[9];
     const int lane = __lane_id();
     if (block < t.ntiles) {
         if (threadIdx.x < 9) {
             const int d = nbr_tile(threadIdx.x, block, t);
             lc[threadIdx.x] = 0;
-            run0[threadIdx.x] = new_off[d] + stay[d] + nbr[9 * block + threadIdx.x];
+            run0[threadIdx.x] = new_off[d] + stay[d] + nbr[ZK_NBR * block + threadIdx.x];
         }
-        /* wave w owns the contiguous piece [wlo, whi) of the tile */
-        const int lo = old_off[block], hi = old_off[block + 1];
-        const int nw = blockDim.x / ZK_WAVE, w = threadIdx.x / ZK_WAVE;
-        const int piece = (hi - lo + nw - 1) / nw;
-        const int wlo = min(hi, lo + w * piece), whi = min(hi, wlo + piece);
-        int nst = 0;
-        for (int i = wlo + lane; i < whi; i += ZK_WAVE) nst += tile_key(src.ix[i], src.iy[i], t) == block;
-        for (int o = ZK_WAVE / 2; o > 0; o >>= 1) nst += __shfl_down(nst, o, ZK_WAVE);
-        if (lane == 0) wst[w] = nst;
-        __syncthreads();
+        const int nw = ZK_SORT_THREADS / ZK_WAVE, w = threadIdx.x / ZK_WAVE;
+        int wlo, whi;
+        wave_piece(old_off[block], old_off[block + 1], w, nw, &wlo, &whi);
         int placed = new_off[block];
-        for (int k = 0; k < w; k++) placed += wst[k];
+        for (int k = 0; k < w; k++) placed += nbr[ZK_NBR * block + 9 + k];
+        __syncthreads();
         const unsigned long long lt = (1ull << lane) - 1ull;
-        for (int base = wlo; base < whi; base += ZK_WAVE) {
-            const int i = base + lane;
-            int ix = 0, iy = 0, key = -1;
-            if (i < whi) {
-                ix = src.ix[i];
-                iy = src.iy[i];
-                key = tile_key(ix, iy, t);
+        for (int base = wlo; base < whi; base += ZK_SORT_UNROLL * ZK_WAVE) {
+            int cx[ZK_SORT_UNROLL], cy[ZK_SORT_UNROLL];
+            float x[ZK_SORT_UNROLL], y[ZK_SORT_UNROLL], ux[ZK_SORT_UNROLL], uy[ZK_SORT_UNROLL], uz[ZK_SORT_UNROLL];
+            int64_t id[ZK_SORT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < ZK_SORT_UNROLL; u++) {
+                const int i = base + u * ZK_WAVE + lane;
+                const bool in = i < whi;
+                cx[u] = in ? src.ix[i] : ZK_DEAD;
+                cy[u] = in ? src.iy[i] : 0;
+                x[u] = in ? src.x[i] : 0.f;
+                y[u] = in ? src.y[i] : 0.f;
+                ux[u] = in ? src.ux[i] : 0.f;
+                uy[u] = in ? src.uy[i] : 0.f;
+                uz[u] = in ? src.uz[i] : 0.f;
+                id[u] = (in && src.id) ? src.id[i] : 0;
             }
-            const bool st = key == block;
-            const unsigned long long ms = __ballot(st);
-            int d = placed + __popcll(ms & lt);
-            placed += __popcll(ms);
-            const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
-            const int r = wave_slot_add(slot, lc);
-            if (slot >= 0) d = run0[slot] + r;
-            const bool far = key >= 0 && !st && slot < 0;
-            const int dm = zk_run_reserve(far ? key : -1, cursor);
-            if (far) d = dm;
-            if (key >= 0) copy_part(src, dst, i, d, ix, iy);
+#pragma unroll
+            for (int u = 0; u < ZK_SORT_UNROLL; u++) {
+                const int key = tile_key(cx[u], cy[u], t);
+                const bool st = key == block;
+                const unsigned long long ms = __ballot(st);
+                int d = placed + __popcll(ms & lt);
+                placed += __popcll(ms);
+                const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
+                const int r = wave_slot_add(slot, lc);
+                if (slot >= 0) d = run0[slot] + r;
+                const bool far = key >= 0 && !st && slot < 0;
+                const int dm = zk_run_reserve(far ? key : -1, cursor);
+                if (far) d = dm;
+                if (key >= 0) {
+                    dst.ix[d] = cx[u];
+                    dst.iy[d] = cy[u];
+                    dst.x[d] = x[u];
+                    dst.y[d] = y[u];
+                    dst.ux[d] = ux[u];
+                    dst.uy[d] = uy[u];
+                    dst.uz[d] = uz[u];
+                    if (src.id) dst.id[d] = id[u];
+                }
+            }
         }
     } else {
         const int n0 = old_off[t.ntiles], n1 = old_off[t.ntiles + 1];

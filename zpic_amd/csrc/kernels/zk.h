@@ -89,7 +89,7 @@ int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr,
                    int np_upper, int spread, zk_stream_t s);
 /* re-sort of a tile-sorted input: old_off (ntiles + 2, the last entry
  * scratch) -> new_off (ntiles + 1); move and tailn (ntiles, zero on entry
- * and exit), stay and cursor (ntiles), nbr (9 ntiles) */
+ * and exit), stay and cursor (ntiles), nbr (13 ntiles) */
 int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *move, int *stay, int *nbr,
                   int *tailn, int *old_off, int *new_off, int *cursor, const zk_tiles *t, int tail_upper,
                   zk_stream_t s);
