@@ -22,17 +22,15 @@ __device__ inline int zk_xcd_item(int b, int n)
     return x * base + (x < rem ? x : rem) + k;
 }
 
-/* Float to 64-bit fixed point: trunc(v * scale) for a power-of-two scale.
- * x = v * scale is exact; hi = trunc(x / 2^31) and the remainder
- * lo = x - 2^31 hi (exact in float, |lo| < 2^31) convert with one
- * instruction each, and trunc(x) = 2^31 hi + trunc(lo).  Valid for
- * |x| < 2^55. */
+/* Float to 64-bit fixed point: v * scale rounded to the nearest integer,
+ * for a power-of-two scale and |v * scale| < 2^51.  The product is exact in
+ * FP64; adding 1.5 * 2^52 in the same fused operation rounds it to an
+ * integer held in the low mantissa bits, and subtracting the bit pattern
+ * of 1.5 * 2^52 (whose low word is zero) leaves its two's complement. */
 __device__ inline unsigned long long zk_fx(float v, float scale)
 {
-    const float x = v * scale;
-    const float hi = truncf(x * 4.656612873077393e-10f); /* x / 2^31 */
-    const float lo = x - hi * 2147483648.0f;             /* exact */
-    return (unsigned long long)(((long long)(int)hi << 31) + (long long)(int)lo);
+    const double r = __builtin_fma((double)v, (double)scale, 6755399441055744.0);
+    return (unsigned long long)__double_as_longlong(r) - 0x4338000000000000ull;
 }
 
 /* Floor modulo for periodic wrap. */
