@@ -174,16 +174,25 @@ __device__ inline void deposit(const D &J, int ix, int iy, int di, int dj, float
         n = 2;
     }
     if (dj != 0) {
+        /* one cut, of the segment that crosses in y: s0 when it is the only
+         * one or when it crosses (the x-cut one then lies beyond, shifted
+         * by dj), else s1.  The cut is written once so that lanes taking
+         * different cases share its instructions. */
+        const bool cut1 = n == 2 && !(s0.y1 < 0.0f || s0.y1 >= 1.0f);
+        Seg c = cut1 ? s1 : s0, tail;
+        ysplit(c, tail, dj);
         if (n == 1) {
-            ysplit(s0, s1, dj);
-        } else if (s0.y1 < 0.0f || s0.y1 >= 1.0f) {
-            /* the first segment crosses in y; the x-cut one lies beyond */
-            ysplit(s0, s2, dj);
+            s0 = c;
+            s1 = tail;
+        } else if (!cut1) {
+            s0 = c;
+            s2 = tail;
             s1.y0 -= dj;
             s1.y1 -= dj;
             s1.iy += dj;
         } else {
-            ysplit(s1, s2, dj);
+            s1 = c;
+            s2 = tail;
         }
         n++;
     }
