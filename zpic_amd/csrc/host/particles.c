@@ -158,7 +158,9 @@ static int inject_positions(t_species *s, const int r[2][2], t_part *out)
 /* Thermal momenta with the per-cell mean removed, plus the fluid drift
  * (em2d/particles.c:85-149).  The cell key uses nx[1] as the row stride,
  * as the reference does. */
-static void inject_momenta(const t_species *s, t_part *p, int n)
+/* Momenta of n injected particles (em2d/particles.c:85-149); w: scratch of
+ * the cell means kept between calls (NULL: a temporary one). */
+static void inject_momenta(const t_species *s, t_part *p, int n, zpic_inj_scratch *w)
 {
     for (int i = 0; i < n; i++) {
         p[i].ux = s->uth[0] * rand_norm();
@@ -167,10 +169,27 @@ static void inject_momenta(const t_species *s, t_part *p, int n)
     }
     /* the row stride is nx[1], as in the reference; the arrays cover every
      * index it makes (the reference's nx[0] * nx[1] arrays are overrun
-     * when nx[1] > nx[0]) */
+     * when nx[1] > nx[0]).  Only the cells that hold new particles are
+     * used, so only those are cleared and scaled (the reference clears
+     * and scales every cell, which gives the same values for these). */
     const int cells = s->nx[1] * (s->nx[0] > s->nx[1] ? s->nx[0] : s->nx[1]);
-    float3 *mean = calloc((size_t)cells, sizeof(float3));
-    int *count = calloc((size_t)cells, sizeof(int));
+    zpic_inj_scratch local = {NULL, NULL, 0};
+    if (!w) w = &local;
+    if (w->cells < cells) {
+        free(w->mean);
+        free(w->count);
+        w->mean = malloc(sizeof(float3) * (size_t)cells);
+        w->count = malloc(sizeof(int) * (size_t)cells);
+        if (!w->mean || !w->count) zpic_fatal("inject_momenta: out of host memory");
+        w->cells = cells;
+    }
+    float3 *mean = w->mean;
+    int *count = w->count;
+    for (int i = 0; i < n; i++) {
+        const int c = p[i].ix + s->nx[1] * p[i].iy;
+        mean[c].x = mean[c].y = mean[c].z = 0.0f;
+        count[c] = 0;
+    }
     for (int i = 0; i < n; i++) {
         const int c = p[i].ix + s->nx[1] * p[i].iy;
         mean[c].x += p[i].ux;
@@ -178,11 +197,15 @@ static void inject_momenta(const t_species *s, t_part *p, int n)
         mean[c].z += p[i].uz;
         count[c] += 1;
     }
-    for (int c = 0; c < cells; c++) {
-        const float w = (count[c] > 0) ? 1.0f / count[c] : 0;
-        mean[c].x *= w;
-        mean[c].y *= w;
-        mean[c].z *= w;
+    for (int i = 0; i < n; i++) {
+        const int c = p[i].ix + s->nx[1] * p[i].iy;
+        if (count[c] > 0) { /* scale each cell once; the sign marks it done */
+            const float wgt = 1.0f / count[c];
+            mean[c].x *= wgt;
+            mean[c].y *= wgt;
+            mean[c].z *= wgt;
+            count[c] = -count[c];
+        }
     }
     for (int i = 0; i < n; i++) {
         const int c = p[i].ix + s->nx[1] * p[i].iy;
@@ -190,8 +213,10 @@ static void inject_momenta(const t_species *s, t_part *p, int n)
         p[i].uy += s->ufl[1] - mean[c].y;
         p[i].uz += s->ufl[2] - mean[c].z;
     }
-    free(count);
-    free(mean);
+    if (w == &local) {
+        free(local.mean);
+        free(local.count);
+    }
 }
 
 void spec_grow_buffer(t_species *s, const int size)
@@ -253,7 +278,7 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
     const int r[2][2] = {{0, nx[0] - 1}, {0, nx[1] - 1}};
     spec_grow_buffer(s, inject_bound(s, r));
     s->np = inject_positions(s, r, s->part);
-    inject_momenta(s, s->part, s->np);
+    inject_momenta(s, s->part, s->np, NULL);
     s->n_sort = 16;
 
     struct zpic_dev_species *d = calloc(1, sizeof(*d));
@@ -300,6 +325,14 @@ static void species_free_device(struct zpic_dev_species *d)
     zpic_dfree(d->epart);
     zpic_dfree(d->stage);
     zpic_dfree(d->ctr);
+    for (int k = 0; k < ZPIC_PIN_SLOTS; k++) {
+        if (d->pin_ev[k]) zpic_event_wait(d->pin_ev[k]);
+        zpic_event_free(d->pin_ev[k]);
+        zpic_pinned_free(d->pin[k]);
+        d->pin_ev[k] = d->pin[k] = NULL;
+        d->pin_cap[k] = 0;
+    }
+    d->pin_next = 0;
     d->tile_off = d->cnt = d->cursor = NULL;
     d->epart = NULL;
     d->stage = NULL;
@@ -328,6 +361,8 @@ void spec_delete(t_species *s)
         if (*pp) *pp = d->next_live;
         pthread_mutex_unlock(&g_live_lock);
         free(d->host_ids);
+        free(d->inj.mean);
+        free(d->inj.count);
         free(d);
         s->dev = NULL;
     }
@@ -587,25 +622,37 @@ int spec_overflow_count(t_species *s)
  * time step (em2d/particles.c:1104-1269)
  * ---------------------------------------------------------------------- */
 /* Inject the rightmost column after a window move (em2d/particles.c:619-640),
- * appending to the unsorted tail on the device. */
+ * appending to the unsorted tail on the device.  The column is built in a
+ * page-locked slot and copied without waiting: the host waits only when it
+ * reuses a slot whose copy has not run yet. */
 static void inject_column(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
     const int r[2][2] = {{s->nx[0] - 1, s->nx[0] - 1}, {0, s->nx[1] - 1}};
     const int bound = inject_bound(s, r);
-    t_part *tmp = malloc(sizeof(t_part) * (size_t)(bound > 0 ? bound : 1));
+    const int k = d->pin_next;
+    d->pin_next = (k + 1) % ZPIC_PIN_SLOTS;
+    if (!d->pin_ev[k]) d->pin_ev[k] = zpic_event_new();
+    zpic_event_wait(d->pin_ev[k]);
+    const size_t need = sizeof(t_part) * (size_t)(bound > 0 ? bound : 1);
+    if (need > d->pin_cap[k]) {
+        zpic_pinned_free(d->pin[k]);
+        d->pin[k] = zpic_pinned_alloc(need);
+        d->pin_cap[k] = need;
+    }
+    t_part *tmp = (t_part *)d->pin[k];
     const int n = inject_positions(s, r, tmp);
-    inject_momenta(s, tmp, n);
+    inject_momenta(s, tmp, n, &d->inj);
     if (n > 0) {
         grow_sets(d, d->np_upper + n);
         ensure_stage(d, sizeof(t_part) * (size_t)n);
-        zpic_h2d(d->stage, tmp, sizeof(t_part) * (size_t)n);
+        zpic_h2d_async(d->stage, tmp, sizeof(t_part) * (size_t)n);
+        zpic_event_record(d->pin_ev[k]);
         ZCHECK(zk_unpack(d->stage, n, &d->set[d->cur], d->next_id, d->ctr, 1, zpic_stream()));
         d->next_id += n;
         d->np_upper += n;
         d->tail_upper += n;
     }
-    free(tmp);
 }
 
 void spec_advance(t_species *s, t_emf *emf, t_current *cur)
@@ -667,7 +714,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
      * left their tile windows take the slow global path, so the tiles are
      * also re-sorted early when that path carries too many pushes (the
      * order of the particles is not part of the results) */
-    int sort = (s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort > 0;
+    int sort = (s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort >= ZPIC_MOVE_SORT;
     d->steps_since_sort++;
     if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
         zk_spec_ctr c;
@@ -692,7 +739,7 @@ void spec_move_window(t_species *s)
         const int r[2][2] = {{s->nx[0] - 1, s->nx[0] - 1}, {0, s->nx[1] - 1}};
         spec_grow_buffer(s, s->np + inject_bound(s, r));
         const int n = inject_positions(s, r, s->part + s->np);
-        inject_momenta(s, s->part + s->np, n);
+        inject_momenta(s, s->part + s->np, n, NULL);
         if (s->dev && s->dev->track_ids) {
             struct zpic_dev_species *d = s->dev;
             if (d->host_ids_cap < s->np_max) {

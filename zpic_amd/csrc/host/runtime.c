@@ -299,3 +299,31 @@ void zpic_h2d_async(void *dst, const void *src, size_t bytes)
 {
     if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, zpic_stream()));
 }
+
+void *zpic_pinned_alloc(size_t bytes)
+{
+    void *p = NULL;
+    ZCHECK(hipHostMalloc(&p, bytes > 0 ? bytes : 1, hipHostMallocDefault));
+    return p;
+}
+
+void zpic_pinned_free(void *p)
+{
+    if (p) ZCHECK(hipHostFree(p));
+}
+
+void *zpic_event_new(void)
+{
+    hipEvent_t e;
+    ZCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return (void *)e;
+}
+
+void zpic_event_record(void *ev) { ZCHECK(hipEventRecord((hipEvent_t)ev, zpic_stream())); }
+
+void zpic_event_wait(void *ev) { ZCHECK(hipEventSynchronize((hipEvent_t)ev)); }
+
+void zpic_event_free(void *ev)
+{
+    if (ev) ZCHECK(hipEventDestroy((hipEvent_t)ev));
+}

@@ -24,6 +24,19 @@
  * the previous probe took the global path, the tiles are re-sorted */
 #define ZPIC_SORT_PROBE 4
 #define ZPIC_SORT_OVF_PERMILLE 5
+/* window moves between tile sorts (each shifts every particle one cell
+ * towards its tile's lower x margin) */
+#define ZPIC_MOVE_SORT 2
+/* page-locked staging slots of the injected columns: the host may run this
+ * many window moves ahead of the GPU */
+#define ZPIC_PIN_SLOTS 8
+
+/* host scratch of the per-cell mean momenta removed from injected particles */
+typedef struct {
+    float3 *mean;
+    int *count;
+    int cells;
+} zpic_inj_scratch;
 
 struct zpic_dev_species {
     int ready;            /* device buffers allocated and filled */
@@ -44,6 +57,11 @@ struct zpic_dev_species {
     int epart_cap;
     void *stage;          /* device AoS staging for uploads/downloads */
     size_t stage_cap;
+    void *pin[ZPIC_PIN_SLOTS];   /* page-locked staging of injected columns */
+    size_t pin_cap[ZPIC_PIN_SLOTS];
+    void *pin_ev[ZPIC_PIN_SLOTS]; /* recorded after the copy out of each slot */
+    int pin_next;
+    zpic_inj_scratch inj; /* host scratch of the injected columns' cell means */
     int np_upper;         /* upper bound of slots in use */
     int tail_upper;       /* upper bound of unsorted slots */
     int moved_since_sort; /* window moves since the last sort */

@@ -27,6 +27,15 @@ void zpic_d2d_async(void *dst, const void *src, size_t bytes); /* on the stream 
 void zpic_sync(void);
 void zpic_h2d_async(void *dst, const void *src, size_t bytes); /* host buffer must outlive the copy */
 
+/* page-locked host memory (copies from it run asynchronously) and events
+ * on the library's stream that the host can wait for */
+void *zpic_pinned_alloc(size_t bytes);
+void zpic_pinned_free(void *p);
+void *zpic_event_new(void);
+void zpic_event_record(void *ev);
+void zpic_event_wait(void *ev); /* returns at once for an event never recorded */
+void zpic_event_free(void *ev);
+
 /* stream capture into a HIP graph; zpic_capture_end returns the
  * instantiated graph, replayed on the stream by zpic_graph_launch */
 void zpic_capture_begin(void);
