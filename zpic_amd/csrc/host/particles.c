@@ -714,7 +714,9 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
      * left their tile windows take the slow global path, so the tiles are
      * also re-sorted early when that path carries too many pushes (the
      * order of the particles is not part of the results) */
-    int sort = (s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort >= ZPIC_MOVE_SORT;
+    /* a window move shifts every particle one cell towards its tile's lower
+     * x margin: after H moves the tiles are sorted again */
+    int sort = (s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort >= (d->t.H > 1 ? d->t.H : 1);
     d->steps_since_sort++;
     if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
         zk_spec_ctr c;

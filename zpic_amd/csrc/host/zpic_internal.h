@@ -24,9 +24,6 @@
  * the previous probe took the global path, the tiles are re-sorted */
 #define ZPIC_SORT_PROBE 4
 #define ZPIC_SORT_OVF_PERMILLE 5
-/* window moves between tile sorts (each shifts every particle one cell
- * towards its tile's lower x margin) */
-#define ZPIC_MOVE_SORT 2
 /* page-locked staging slots of the injected columns: the host may run this
  * many window moves ahead of the GPU */
 #define ZPIC_PIN_SLOTS 8
