@@ -23,7 +23,7 @@
  * read; if more than ZPIC_SORT_OVF_PERMILLE per mille of the pushes since
  * the previous probe took the global path, the tiles are re-sorted */
 #define ZPIC_SORT_PROBE 4
-#define ZPIC_SORT_OVF_PERMILLE 5
+#define ZPIC_SORT_OVF_PERMILLE 2
 /* page-locked staging slots of the injected columns: the host may run this
  * many window moves ahead of the GPU */
 #define ZPIC_PIN_SLOTS 8
