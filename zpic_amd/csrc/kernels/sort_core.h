@@ -10,10 +10,15 @@
 namespace zk {
 
 
+/* floor(a / d) for 0 <= a < 2^22, with inv = 1.0f / d (correctly rounded):
+ * a + 0.5 lies at least 0.5 from a multiple of d, farther than the product
+ * can be off (relative 2^-23, i.e. below 0.5 / d for a < 2^22) */
+__device__ inline int zk_div(int a, float inv) { return (int)(((float)a + 0.5f) * inv); }
+
 __device__ inline int tile_key(int ix, int iy, const zk_tiles &t)
 {
     if (ix == ZK_DEAD) return -1;
-    return ix / t.TX + (iy / t.TY) * t.ntx;
+    return zk_div(ix, 1.0f / t.TX) + zk_div(iy, 1.0f / t.TY) * t.ntx;
 }
 
 __device__ inline void count_body(const int *__restrict__ ix, const int *__restrict__ iy, const zk_spec_ctr *ctr,
@@ -144,7 +149,9 @@ __device__ inline void wave_piece(int a, int b, int w, int nw, int *lo, int *hi)
  * b itself, -1 when not adjacent */
 __device__ inline int nbr_slot(int key, int b, const zk_tiles &t)
 {
-    int dx = key % t.ntx - b % t.ntx, dy = key / t.ntx - b / t.ntx;
+    const float inv = 1.0f / t.ntx;
+    const int ky = zk_div(key, inv), by = zk_div(b, inv);
+    int dx = (key - ky * t.ntx) - (b - by * t.ntx), dy = ky - by;
     if (dx > 1) dx -= t.ntx;
     else if (dx < -1) dx += t.ntx;
     if (dy > 1) dy -= t.nty;
