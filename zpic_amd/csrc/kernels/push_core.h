@@ -210,30 +210,31 @@ struct Part {
     float x, y, ux, uy, uz;
 };
 
-/* Particle streams are read and written once per step: nontemporal
- * accesses keep them from evicting the field windows from the caches. */
+/* Particle streams are read and written once per step with ordinary
+ * loads and stores: nontemporal hints on them made the 2048^2 Weibel push
+ * 14 % slower (1.35 against 1.16 ms per launch). */
 __device__ inline Part load_part(const zk_parts &P, int i)
 {
     Part c;
-    c.ix = __builtin_nontemporal_load(&P.ix[i]);
-    c.iy = __builtin_nontemporal_load(&P.iy[i]);
-    c.x = __builtin_nontemporal_load(&P.x[i]);
-    c.y = __builtin_nontemporal_load(&P.y[i]);
-    c.ux = __builtin_nontemporal_load(&P.ux[i]);
-    c.uy = __builtin_nontemporal_load(&P.uy[i]);
-    c.uz = __builtin_nontemporal_load(&P.uz[i]);
+    c.ix = P.ix[i];
+    c.iy = P.iy[i];
+    c.x = P.x[i];
+    c.y = P.y[i];
+    c.ux = P.ux[i];
+    c.uy = P.uy[i];
+    c.uz = P.uz[i];
     return c;
 }
 
 __device__ inline void store_part(const zk_parts &P, int i, const Part &c)
 {
-    __builtin_nontemporal_store(c.ix, &P.ix[i]);
-    __builtin_nontemporal_store(c.iy, &P.iy[i]);
-    __builtin_nontemporal_store(c.x, &P.x[i]);
-    __builtin_nontemporal_store(c.y, &P.y[i]);
-    __builtin_nontemporal_store(c.ux, &P.ux[i]);
-    __builtin_nontemporal_store(c.uy, &P.uy[i]);
-    __builtin_nontemporal_store(c.uz, &P.uz[i]);
+    P.ix[i] = c.ix;
+    P.iy[i] = c.iy;
+    P.x[i] = c.x;
+    P.y[i] = c.y;
+    P.ux[i] = c.ux;
+    P.uy[i] = c.uy;
+    P.uz[i] = c.uz;
 }
 
 /* Motion of one particle whose cell is (ci, cj) in the sources' index
