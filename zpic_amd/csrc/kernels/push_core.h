@@ -390,12 +390,8 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     }
     const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
     if (PAIR == 1) {
-        /* the next particle of this thread is loaded one iteration ahead */
-        Part cn;
-        if (i0 + (int)threadIdx.x < i1) cn = load_part(P, i0 + threadIdx.x);
         for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            Part c = cn;
-            if (i + (int)blockDim.x < i1) cn = load_part(P, i + blockDim.x);
+            Part c = load_part(P, i);
             const int lx = c.ix - tx0, ly = c.iy - ty0;
             const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
             const bool live = c.ix != ZK_DEAD;
