@@ -196,18 +196,36 @@ static struct {
 } g_timer[ZPIC_NTIMERS];
 static int g_ktiming = 0;
 
-static void timer_drain(int w)
+/* accumulate the oldest n recorded intervals and drop them */
+static void timer_drain_n(int w, int n)
 {
-    if (!g_timer[w].used) return;
-    ZCHECK(hipEventSynchronize(g_timer[w].ev[g_timer[w].used - 1][1]));
-    for (int k = 0; k < g_timer[w].used; k++) {
+    if (n <= 0) return;
+    ZCHECK(hipEventSynchronize(g_timer[w].ev[n - 1][1]));
+    for (int k = 0; k < n; k++) {
         float ms = 0.0f;
         ZCHECK(hipEventElapsedTime(&ms, g_timer[w].ev[k][0], g_timer[w].ev[k][1]));
         g_timer[w].total += 1e-3 * ms;
         g_timer[w].count++;
     }
-    g_timer[w].used = 0;
+    /* the drained event pairs move behind the ones still in flight */
+    hipEvent_t keep[ZPIC_EV_PAIRS][2];
+    const int rest = g_timer[w].used - n;
+    for (int k = 0; k < rest; k++) {
+        keep[k][0] = g_timer[w].ev[n + k][0];
+        keep[k][1] = g_timer[w].ev[n + k][1];
+    }
+    for (int k = 0; k < n; k++) {
+        keep[rest + k][0] = g_timer[w].ev[k][0];
+        keep[rest + k][1] = g_timer[w].ev[k][1];
+    }
+    for (int k = 0; k < g_timer[w].used; k++) {
+        g_timer[w].ev[k][0] = keep[k][0];
+        g_timer[w].ev[k][1] = keep[k][1];
+    }
+    g_timer[w].used = rest;
 }
+
+static void timer_drain(int w) { timer_drain_n(w, g_timer[w].used); }
 
 void zpic_timer_begin(int w)
 {
@@ -219,7 +237,9 @@ void zpic_timer_begin(int w)
         }
         g_timer[w].created = 1;
     }
-    if (g_timer[w].used == ZPIC_EV_PAIRS) timer_drain(w);
+    /* full: drain the older half, recorded long enough ago that the host
+     * does not wait for the GPU here */
+    if (g_timer[w].used == ZPIC_EV_PAIRS) timer_drain_n(w, ZPIC_EV_PAIRS / 2);
     ZCHECK(hipEventRecord(g_timer[w].ev[g_timer[w].used][0], s));
     g_timer[w].open = 1;
 }
