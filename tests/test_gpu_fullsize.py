@@ -1,0 +1,73 @@
+"""The time step at the size of the throughput benchmark, 2048 x 2048 cells
+with two electron species of 4 x 4 particles per cell (134,217,728
+particles), checked through properties that hold at any size: the oracle
+is too slow to follow this case, and the small-grid parity tests pin the
+arithmetic itself.
+
+* discrete continuity, (rho1 - rho0)/dt + div J = 0 at FP32 rounding, for
+  every step of a stretch that spans tile sorts (the deposit, the tile
+  windows, the out-of-window path and the periodic fold at full size);
+* the Gauss-law defect div E - rho stays constant (the field solver and
+  the deposit agree);
+* no particle is lost or duplicated by the sorts (particle count and total
+  charge).
+
+No current filter, so that J is the raw deposit the continuity equation
+is about."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from em2d_driver import Em2dRun, weibel
+from test_oracle import continuity_residual, gauss_defect
+from zpic_amd import abi as A
+
+pytestmark = pytest.mark.gpu
+
+NX = 2048
+
+
+def fields(gpu, run):
+    """E and J of the device state (only the grids are downloaded)."""
+    gpu.emf_sync_host(C.byref(run.sim.emf))
+    gpu.current_sync_host(C.byref(run.sim.current))
+    nx = (NX, NX)
+    return (A.grid_view(run.sim.emf.E_buf, nx).copy(), A.grid_view(run.sim.current.J_buf, nx).copy())
+
+
+def rho(run):
+    return sum(run.charge(k).astype(np.float64) for k in range(2))[:NX, :NX]
+
+
+def test_full_size_step_conserves_charge_across_sorts(gpu):
+    cfg = weibel(NX, NX, 4, n_sort=2, m_q2=-1.0)
+    dx = dy = cfg["box"][0] / NX
+    dt = cfg["dt"]
+    print("fullsize: building 2048x2048, 134M particles", flush=True)
+    run = Em2dRun(gpu, cfg)
+    try:
+        run.step(2)
+        E, _ = fields(gpu, run)
+        r = rho(run)
+        q0 = float(run.charge(0)[:NX, :NX].astype(np.float64).sum())
+        gd0 = gauss_defect(E, r, dx, dy)
+        for n in range(5):  # sorts after steps 4 and 6
+            run.step(1)
+            E, J = fields(gpu, run)
+            r1 = rho(run)
+            res, scale = continuity_residual(r, r1, J, dt, dx, dy)
+            print(f"fullsize: step {run.n} continuity {res:.3e} of {scale:.3e}", flush=True)
+            assert res <= 1e-5 * scale
+            r = r1
+        gd1 = gauss_defect(E, r, dx, dy)
+        one = np.abs(run.charge(0)).max()
+        assert np.abs(gd1 - gd0).max() <= 1e-5 * one
+        q1 = float(run.charge(0)[:NX, :NX].astype(np.float64).sum())
+        assert abs(q1 - q0) <= 1e-6 * abs(q0)
+        gpu.sim_sync_host(C.byref(run.sim))
+        assert [int(run.species[k].np) for k in range(2)] == [NX * NX * 16] * 2
+    finally:
+        run.close()
