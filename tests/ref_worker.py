@@ -2,7 +2,7 @@
 process (the reference's generator keeps a cached deviate between runs,
 em2d/random.c:69-70, so every comparison starts from a fresh library).
 
-usage: python tests/ref_worker.py <case>   -> prints a JSON verdict
+usage: python tests/ref_worker.py <case>   -> prints a JSON result
 """
 from __future__ import annotations
 

@@ -60,8 +60,8 @@ def test_oracle_matches_compiled_reference(reference, name):
     against the reference build, each in a fresh process."""
     out = subprocess.run([sys.executable, os.path.join(HERE, "ref_worker.py"), name], check=True,
                          capture_output=True, text=True, timeout=300)
-    verdict = json.loads(out.stdout.strip().splitlines()[-1])
-    assert verdict["mismatch"] == [], verdict
+    result = json.loads(out.stdout.strip().splitlines()[-1])
+    assert result["mismatch"] == [], result
 
 
 # --------------------------------------------------------------------------
