@@ -137,68 +137,61 @@ __device__ inline void deposit_seg(const D &J, const Seg &g, float qnx, float qn
 }
 
 /* Split the motion at cell edges into up to three segments and deposit
- * them in the reference's order (em2d/particles.c:773-924). */
+ * them (em2d/particles.c:773-924).  Each segment is computed as in the
+ * reference; the deposit order differs (the fixed-point windows do not
+ * depend on it): the segment cut in y, the other x-cut segment, then the
+ * piece beyond the y edge. */
 template <class D>
 __device__ inline void deposit(const D &J, int ix, int iy, int di, int dj, float x0, float y0,
                                float dx, float dy, float qnx, float qny, float qvz)
 {
-    Seg s0, s1, s2;
-    s0.x0 = x0;
-    s0.y0 = y0;
-    s0.dx = dx;
-    s0.dy = dy;
-    s0.x1 = x0 + dx;
-    s0.y1 = y0 + dy;
-    s0.qvz = qvz * 0.5f; /* equals (float)(qvz / 2.0): halving is exact */
-    s0.ix = ix;
-    s0.iy = iy;
-    int n = 1;
-    if (di != 0) {
+    Seg a;
+    a.x0 = x0;
+    a.y0 = y0;
+    a.dx = dx;
+    a.dy = dy;
+    a.x1 = x0 + dx;
+    a.y1 = y0 + dy;
+    a.qvz = qvz * 0.5f; /* equals (float)(qvz / 2.0): halving is exact */
+    a.ix = ix;
+    a.iy = iy;
+    const bool xcut = di != 0;
+    Seg b = a; /* used only when xcut */
+    if (xcut) {
         const int edge = (di == 1);
         const float f = (x0 + dx - edge) / dx;
-        s1.x0 = 1 - edge;
-        s1.x1 = (x0 + dx) - di;
-        s1.dx = dx * f;
-        s1.ix = ix + di;
+        b.x0 = 1 - edge;
+        b.x1 = (x0 + dx) - di;
+        b.dx = dx * f;
+        b.ix = ix + di;
         const float yc = y0 + dy * (1.0f - f);
-        s1.y0 = yc;
-        s1.y1 = s0.y1;
-        s1.dy = dy * f;
-        s1.iy = iy;
-        s1.qvz = s0.qvz * f;
-        s0.x1 = edge;
-        s0.dx *= (1.0f - f);
-        s0.dy *= (1.0f - f);
-        s0.y1 = yc;
-        s0.qvz *= (1.0f - f);
-        n = 2;
+        b.y0 = yc;
+        b.y1 = a.y1;
+        b.dy = dy * f;
+        b.iy = iy;
+        b.qvz = a.qvz * f;
+        a.x1 = edge;
+        a.dx *= (1.0f - f);
+        a.dy *= (1.0f - f);
+        a.y1 = yc;
+        a.qvz *= (1.0f - f);
     }
+    /* p: the segment that crosses the y edge (a when it is the only one or
+     * when it crosses, else b); q: the other one, when there are two.  When
+     * a crosses, b lies beyond the y edge: it moves one cell in y. */
+    const bool bcuts = xcut && !(a.y1 < 0.0f || a.y1 >= 1.0f);
+    Seg p = bcuts ? b : a, q = bcuts ? a : b, t;
     if (dj != 0) {
-        /* one cut, of the segment that crosses in y: s0 when it is the only
-         * one or when it crosses (the x-cut one then lies beyond, shifted
-         * by dj), else s1.  The cut is written once so that lanes taking
-         * different cases share its instructions. */
-        const bool cut1 = n == 2 && !(s0.y1 < 0.0f || s0.y1 >= 1.0f);
-        Seg c = cut1 ? s1 : s0, tail;
-        ysplit(c, tail, dj);
-        if (n == 1) {
-            s0 = c;
-            s1 = tail;
-        } else if (!cut1) {
-            s0 = c;
-            s2 = tail;
-            s1.y0 -= dj;
-            s1.y1 -= dj;
-            s1.iy += dj;
-        } else {
-            s1 = c;
-            s2 = tail;
+        if (xcut && !bcuts) {
+            q.y0 -= dj;
+            q.y1 -= dj;
+            q.iy += dj;
         }
-        n++;
+        ysplit(p, t, dj);
     }
-    deposit_seg(J, s0, qnx, qny);
-    if (n > 1) deposit_seg(J, s1, qnx, qny);
-    if (n > 2) deposit_seg(J, s2, qnx, qny);
+    deposit_seg(J, p, qnx, qny);
+    if (xcut) deposit_seg(J, q, qnx, qny);
+    if (dj != 0) deposit_seg(J, t, qnx, qny);
 }
 
 struct Consts {
