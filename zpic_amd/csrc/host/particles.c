@@ -795,30 +795,49 @@ static float pha_quantity(const t_species *s, int i, int q)
 void spec_deposit_pha(const t_species *cs, const int rep_type, const int pha_nx[], const float pha_range[][2],
                       float *buf)
 {
-    /* phase-space density from the host mirror (em2d/particles.c:1569-1632) */
+    /* phase-space density (em2d/particles.c:1569-1632): the reference's
+     * weights and bounds per particle, added into buf.  On the device once
+     * the particles live there; from the host buffer while it is the
+     * authoritative copy (before the first step, or after host edits not
+     * yet uploaded) */
     t_species *s = (t_species *)cs;
-    spec_sync_host(s);
-    const int nrow = pha_nx[0];
+    struct zpic_dev_species *d = s->dev;
+    const int na = pha_nx[0], nb = pha_nx[1];
+    if (na <= 0 || nb <= 0) return;
     const int qa = rep_type & 0x000F, qb = (rep_type & 0x00F0) >> 4;
     const float amin = pha_range[0][0], bmin = pha_range[1][0];
     const float ra = pha_nx[0] / (pha_range[0][1] - pha_range[0][0]);
     const float rb = pha_nx[1] / (pha_range[1][1] - pha_range[1][0]);
-    for (int i = 0; i < s->np; i++) {
-        const float na = (pha_quantity(s, i, qa) - amin) * ra;
-        const float nb = (pha_quantity(s, i, qb) - bmin) * rb;
-        const int ia = (int)(na + 0.5f), ib = (int)(nb + 0.5f);
-        const float wa = na - ia + 0.5f, wb = nb - ib + 0.5f;
-        int c = ia + nrow * ib;
-        if (ib >= 0 && ib < pha_nx[1]) {
-            if (ia >= 0 && ia < pha_nx[0]) buf[c] += (1.0f - wa) * (1.0f - wb) * s->q;
-            if (ia + 1 >= 0 && ia + 1 < pha_nx[0]) buf[c + 1] += wa * (1.0f - wb) * s->q;
+    if (!d || !d->ready || d->host_dirty) {
+        for (int i = 0; i < s->np; i++) {
+            const float va = (pha_quantity(s, i, qa) - amin) * ra;
+            const float vb = (pha_quantity(s, i, qb) - bmin) * rb;
+            const int ia = (int)(va + 0.5f), ib = (int)(vb + 0.5f);
+            const float wa = va - ia + 0.5f, wb = vb - ib + 0.5f;
+            int c = ia + na * ib;
+            if (ib >= 0 && ib < nb) {
+                if (ia >= 0 && ia < na) buf[c] += (1.0f - wa) * (1.0f - wb) * s->q;
+                if (ia + 1 >= 0 && ia + 1 < na) buf[c + 1] += wa * (1.0f - wb) * s->q;
+            }
+            c += na;
+            if (ib + 1 >= 0 && ib + 1 < nb) {
+                if (ia >= 0 && ia < na) buf[c] += (1.0f - wa) * wb * s->q;
+                if (ia + 1 >= 0 && ia + 1 < na) buf[c + 1] += wa * wb * s->q;
+            }
         }
-        c += nrow;
-        if (ib + 1 >= 0 && ib + 1 < pha_nx[1]) {
-            if (ia >= 0 && ia < pha_nx[0]) buf[c] += (1.0f - wa) * wb * s->q;
-            if (ia + 1 >= 0 && ia + 1 < pha_nx[0]) buf[c + 1] += wa * wb * s->q;
-        }
+        return;
     }
+    zpic_species_ensure(s);
+    const size_t n = (size_t)na * (size_t)nb;
+    float *dbuf = zpic_dmalloc(sizeof(float) * n);
+    ZCHECK(zk_fill_f32(dbuf, n, 0.0f, zpic_stream()));
+    ZCHECK(zk_pha(&d->set[d->cur], d->np_upper, d->ctr, qa, qb, s->dx[0], s->dx[1], na, nb, amin, bmin, ra, rb, s->q,
+                  dbuf, zpic_stream()));
+    float *h = malloc(sizeof(float) * n);
+    zpic_d2h(h, dbuf, sizeof(float) * n);
+    zpic_dfree(dbuf);
+    for (size_t k = 0; k < n; k++) buf[k] += h[k];
+    free(h);
 }
 
 /* em2d/particles.c:1339-1416: positions (from the box corner) and

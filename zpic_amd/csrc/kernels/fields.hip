@@ -110,7 +110,73 @@ __global__ void charge_kernel(zk_parts p, const zk_spec_ctr *ctr, float q, int n
     }
 }
 
+/* phase-space quantity code (X1 1, X2 2, U1 4, U2 5, U3 6) of particle i,
+ * as em2d/particles.c:1569-1632 forms it */
+__device__ inline float pha_quantity(const zk_parts &p, int i, int q, float dx0, float dx1)
+{
+    switch (q) {
+    case 1: return (p.x[i] + p.ix[i]) * dx0;
+    case 2: return (p.y[i] + p.iy[i]) * dx1;
+    case 4: return p.ux[i];
+    case 5: return p.uy[i];
+    case 6: return p.uz[i];
+    }
+    return 0.0f;
+}
+
+/* Bilinear phase-space density of quantities (qa, qb): the reference's
+ * weights and bounds per particle, summed into a block-private LDS copy of
+ * the grid when it fits (lds = 1), else straight into the global grid. */
+__global__ void pha_kernel(zk_parts p, const zk_spec_ctr *ctr, int qa, int qb, float dx0, float dx1, int na, int nb,
+                           float amin, float bmin, float ra, float rb, float q, float *out, int lds)
+{
+    extern __shared__ float hist[];
+    const int nbins = na * nb;
+    float *h = lds ? hist : out;
+    if (lds) {
+        for (int m = threadIdx.x; m < nbins; m += blockDim.x) hist[m] = 0.0f;
+        __syncthreads();
+    }
+    const int n = ctr->np_total;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (p.ix[i] == ZK_DEAD) continue;
+        const float va = (pha_quantity(p, i, qa, dx0, dx1) - amin) * ra;
+        const float vb = (pha_quantity(p, i, qb, dx0, dx1) - bmin) * rb;
+        const int ia = (int)(va + 0.5f), ib = (int)(vb + 0.5f);
+        const float wa = va - ia + 0.5f, wb = vb - ib + 0.5f;
+        int c = ia + na * ib;
+        if (ib >= 0 && ib < nb) {
+            if (ia >= 0 && ia < na) atomicAdd(&h[c], (1.0f - wa) * (1.0f - wb) * q);
+            if (ia + 1 >= 0 && ia + 1 < na) atomicAdd(&h[c + 1], wa * (1.0f - wb) * q);
+        }
+        c += na;
+        if (ib + 1 >= 0 && ib + 1 < nb) {
+            if (ia >= 0 && ia < na) atomicAdd(&h[c], (1.0f - wa) * wb * q);
+            if (ia + 1 >= 0 && ia + 1 < na) atomicAdd(&h[c + 1], wa * wb * q);
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int m = threadIdx.x; m < nbins; m += blockDim.x)
+            if (hist[m] != 0.0f) unsafeAtomicAdd(&out[m], hist[m]);
+    }
+}
+
 } // namespace
+
+extern "C" int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, int qa, int qb, float dx0, float dx1,
+                      int na, int nb, float amin, float bmin, float ra, float rb, float q, float *out, zk_stream_t s)
+{
+    if (na <= 0 || nb <= 0) return 0;
+    const size_t lds = (size_t)na * nb * sizeof(float);
+    const int use_lds = lds <= 48 * 1024;
+    long long b = (np_upper + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > (use_lds ? 1024 : 4096)) b = use_lds ? 1024 : 4096;
+    hipLaunchKernelGGL(pha_kernel, dim3((unsigned)b), dim3(256), use_lds ? lds : 0, s, *p, ctr, qa, qb, dx0, dx1, na,
+                       nb, amin, bmin, ra, rb, q, out, use_lds);
+    ZK_LAUNCH_CHECK();
+}
 
 extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo, int nx, int ny,
                              int nrow, float dt, float dx, float dy, zk_stream_t s)

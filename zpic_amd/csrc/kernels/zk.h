@@ -125,6 +125,11 @@ int zk_emf_energy(const float *E, const float *B, int nx, int ny, int nrow,
                          double *partial, int nblk, double *out6, double scale, zk_stream_t s);
 int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, float q,
                      int nx, float *rho, zk_stream_t s);
+/* out (na x nb, row length na) += phase-space density of quantities qa, qb
+ * (em2d/particles.c:1569-1632 weights): amin/bmin the lower range bounds,
+ * ra/rb the bins per unit */
+int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, int qa, int qb, float dx0, float dx1, int na,
+           int nb, float amin, float bmin, float ra, float rb, float q, float *out, zk_stream_t s);
 
 /* ---- batches of independent simulations (batch.hip) ---- */
 #define ZK_MAX_PASSES 16
