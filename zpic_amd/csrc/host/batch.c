@@ -26,7 +26,6 @@
  * particles, within [4, ZPIC_GLOBAL_BLOCKS] */
 #define BATCH_PARTS_PER_GLOBAL_BLOCK 16384
 /* particles per block of the sort's count and scatter passes */
-#define BATCH_PARTS_PER_SORT_BLOCK 4096
 
 typedef struct {
     int off, n; /* items [off, off + n) of the table, in pairs */
@@ -45,7 +44,7 @@ struct zpic_batch {
     int *ditems;        /* device item table (int pairs) */
     item_range fill, node, yee, guard, sort;
     item_range *tile, *glob; /* per species index */
-    int *gblocks, *sblocks;  /* per species: blocks of the out-of-window pass and of the sort */
+    int *gblocks;            /* per species: blocks of the out-of-window pass */
     int *ntiles;             /* per simulation, at creation */
     size_t lds;
     int waves, fixed;
@@ -129,10 +128,14 @@ static void fill_spec(zpic_batch *b, int k)
     S->nparts = d->t.ntiles + a->global_blocks;
     S->iter = s->iter;
     S->n_sort = s->n_sort;
-    S->sort_blocks = b->sblocks[k];
-    S->off = d->tile_off;
+    S->offs[0] = d->tile_off;
+    S->offs[1] = d->tile_off2;
+    S->ocur = 0;
     S->cnt = d->cnt;
     S->cursor = d->cursor;
+    S->stay = d->stay;
+    S->nbr = d->nbr;
+    S->tailn = d->tailn;
     const float qm = s->q * s->m_q;
     S->qm = qm;
     S->dx0 = s->dx[0];
@@ -153,6 +156,8 @@ static void refresh(zpic_batch *b)
         zpic_species_ensure(s);
         struct zpic_dev_species *d = s->dev;
         if (d->moved_since_sort) zpic_fatal("batch: species '%s' has an unsorted window shift", s->name);
+        /* the batched re-sort keeps tile ranges only: no unsorted tail */
+        if (d->tail_upper > 0) zpic_spec_sort(s);
         if (d->epart_cap < d->t.ntiles + ZPIC_GLOBAL_BLOCKS) zpic_fatal("batch: energy partials too small");
         zpic_current_ensure(&b->sims[b->spec_sim[k]]->current, &d->t);
     }
@@ -250,10 +255,11 @@ static void build_items(zpic_batch *b, table *T)
         w[k] = (double)b->spec[k]->np;
     }
     qsort(order, (size_t)b->nspecs, sizeof(int), by_weight);
+    /* the re-sort: one workgroup per tile of every species */
     b->sort.off = T->n;
     for (int q = 0; q < b->nspecs; q++) {
         const int k = order[q];
-        for (int g = 0; g < b->hspecs[k].sort_blocks; g++) put(T, k, g);
+        for (int t = 0; t < b->hspecs[k].a.t.ntiles; t++) put(T, k, t);
     }
     b->sort.n = T->n - b->sort.off;
     free(order);
@@ -325,11 +331,9 @@ zpic_batch *zpic_batch_new(t_simulation *const *sims, int n)
     b->fixed = zpic_fixed_deposit();
     /* block counts of the particle passes are fixed with the item tables */
     b->gblocks = malloc(sizeof(int) * (size_t)ns);
-    b->sblocks = malloc(sizeof(int) * (size_t)ns);
     for (int k = 0; k < b->nspecs; k++) {
         const int np = b->spec[k]->np > 0 ? b->spec[k]->np : 0;
         b->gblocks[k] = clampi(np / BATCH_PARTS_PER_GLOBAL_BLOCK, 4, ZPIC_GLOBAL_BLOCKS);
-        b->sblocks[k] = clampi((np + BATCH_PARTS_PER_SORT_BLOCK - 1) / BATCH_PARTS_PER_SORT_BLOCK, 1, 8192);
     }
 
     refresh(b);
@@ -387,7 +391,15 @@ void zpic_batch_run(zpic_batch *b, int steps)
         for (int k = 0; k < b->nspecs; k++) {
             t_species *s = b->spec[k];
             s->iter++;
-            if (s->n_sort > 0 && s->iter % s->n_sort == 0) s->dev->cur ^= 1;
+            if (s->n_sort > 0 && s->iter % s->n_sort == 0) {
+                /* the re-sort wrote the other particle set and tile ranges */
+                struct zpic_dev_species *d = s->dev;
+                int *tmp = d->tile_off;
+                d->tile_off = d->tile_off2;
+                d->tile_off2 = tmp;
+                d->cur ^= 1;
+                d->steps_since_sort = 0;
+            }
         }
         for (int i = 0; i < b->nsims; i++) {
             t_simulation *sim = b->sims[i];
@@ -424,7 +436,6 @@ void zpic_batch_delete(zpic_batch *b)
     free(b->tile);
     free(b->glob);
     free(b->gblocks);
-    free(b->sblocks);
     free(b->ntiles);
     free(b->hsims);
     free(b->hspecs);

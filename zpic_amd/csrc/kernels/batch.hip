@@ -29,6 +29,7 @@ __device__ inline zk_push_args spec_args(const zk_bspec &S, const zk_bsim &M)
 {
     zk_push_args a = S.a;
     a.p = S.set[S.pcur];
+    a.tile_off = S.offs[S.ocur];
     a.E = M.E[M.ecur];
     a.B = M.B[M.ecur];
     return a;
@@ -108,33 +109,37 @@ __global__ void guards_batch(const zk_bsim *sims, const int *items)
                items[2 * blockIdx.x + 1] * blockDim.x + threadIdx.x);
 }
 
+/* The re-sort of the single-simulation path (zk_sort_tiles: stayers in
+ * order, movers by neighbour slot), one workgroup per (species, tile)
+ * item; batch members have no unsorted tail (no moving window). */
 __global__ void snapshot_batch(const zk_bspec *specs)
 {
     const zk_bspec &S = specs[blockIdx.x];
-    if (sorts_now(S)) S.off[S.a.t.ntiles + 1] = S.a.ctr->np_total;
+    if (sorts_now(S)) S.offs[S.ocur][S.a.t.ntiles + 1] = S.a.ctr->np_total;
 }
 
-__global__ void count_batch(const zk_bspec *specs, const int *items)
+__global__ void __launch_bounds__(ZK_SORT_THREADS) count_batch(const zk_bspec *specs, const int *items)
 {
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     if (!sorts_now(S)) return;
     const zk_parts &p = S.set[S.pcur];
-    count_body(p.ix, p.iy, S.a.ctr, S.cnt, S.a.t, items[2 * blockIdx.x + 1], S.sort_blocks);
+    count_stay_body(p.ix, p.iy, S.offs[S.ocur], S.stay, S.cnt, S.nbr, S.tailn, S.a.t, items[2 * blockIdx.x + 1],
+                    S.a.t.ntiles);
 }
 
 __global__ void __launch_bounds__(1024) scan_batch(const zk_bspec *specs)
 {
     const zk_bspec &S = specs[blockIdx.x];
     if (!sorts_now(S)) return;
-    scan_body(S.cnt, S.off, S.cursor, S.a.ctr, S.a.t.ntiles);
+    scan_stay_body(S.stay, S.cnt, S.tailn, S.offs[1 - S.ocur], S.cursor, S.a.ctr, S.a.t.ntiles);
 }
 
-__global__ void scatter_batch(const zk_bspec *specs, const int *items)
+__global__ void __launch_bounds__(ZK_SORT_THREADS) scatter_batch(const zk_bspec *specs, const int *items)
 {
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     if (!sorts_now(S)) return;
-    scatter_body(S.set[S.pcur], S.set[1 - S.pcur], S.off + S.a.t.ntiles + 1, S.cursor, S.off, S.a.t, 0,
-                 items[2 * blockIdx.x + 1], S.sort_blocks);
+    scatter_stay_body(S.set[S.pcur], S.set[1 - S.pcur], S.offs[S.ocur], S.offs[1 - S.ocur], S.stay, S.nbr, S.cursor,
+                      S.a.t, items[2 * blockIdx.x + 1], S.a.t.ntiles);
 }
 
 /* Advance the buffer indices of every simulation and species so the next
@@ -147,7 +152,10 @@ __global__ void end_step_batch(zk_bsim *sims, int nsims, zk_bspec *specs, int ns
         sims[s].jcur = (sims[s].jcur + sims[s].npass) & 1;
     }
     for (int k = threadIdx.x; k < nspecs; k += blockDim.x) {
-        if (sorted && sorts_now(specs[k])) specs[k].pcur ^= 1;
+        if (sorted && sorts_now(specs[k])) {
+            specs[k].pcur ^= 1;
+            specs[k].ocur ^= 1;
+        }
         specs[k].iter++;
     }
 }

@@ -160,8 +160,10 @@ typedef struct zk_bsim {
 typedef struct zk_bspec {
     zk_push_args a;
     zk_parts set[2];
-    int pcur, sim, nparts, iter, n_sort, sort_blocks;
-    int *off, *cnt, *cursor; /* off is a.tile_off, writable */
+    int pcur, sim, nparts, iter, n_sort;
+    int *offs[2];            /* tile ranges: offs[ocur] is the push's, the re-sort writes the other */
+    int ocur;
+    int *cnt, *cursor, *stay, *nbr, *tailn; /* re-sort scratch (zk_sort_tiles) */
     double qm, dx0, dx1;
 } zk_bspec;
 
