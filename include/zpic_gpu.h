@@ -43,9 +43,10 @@ void zpic_sim_set_tiling(t_simulation *sim, int tx, int ty, int halo);
  * identical generator states. */
 void zpic_rand_clear(void);
 
-/* Variant of the push kernel: 0 (default) keeps every value in registers,
- * 8 requests 8 waves per SIMD with a few register spills, 2 pushes two
- * particles per thread and iteration. */
+/* Variant of the float-atomic push kernel (zpic_set_fixed_deposit(0)):
+ * 0 (default) keeps every value in registers, 8 requests 8 waves per SIMD
+ * with a few register spills, 2 pushes two particles per thread and
+ * iteration.  The fixed-point push (the default) has one variant. */
 void zpic_set_push_waves(int waves);
 
 /* Current deposit of the push: 1 (default; environment variable
