@@ -24,7 +24,6 @@
 
 /* blocks per species of the out-of-window pass: one per this many
  * particles, within [4, ZPIC_GLOBAL_BLOCKS] */
-#define BATCH_PARTS_PER_GLOBAL_BLOCK 16384
 /* particles per block of the sort's count and scatter passes */
 
 typedef struct {
@@ -50,8 +49,6 @@ struct zpic_batch {
     int waves, fixed;
     void *graph[2];     /* 0: plain step, 1: step with the tile sort */
 };
-
-static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 /* ---- descriptors ------------------------------------------------------- */
 
@@ -331,10 +328,9 @@ zpic_batch *zpic_batch_new(t_simulation *const *sims, int n)
     b->fixed = zpic_fixed_deposit();
     /* block counts of the particle passes are fixed with the item tables */
     b->gblocks = malloc(sizeof(int) * (size_t)ns);
-    for (int k = 0; k < b->nspecs; k++) {
-        const int np = b->spec[k]->np > 0 ? b->spec[k]->np : 0;
-        b->gblocks[k] = clampi(np / BATCH_PARTS_PER_GLOBAL_BLOCK, 4, ZPIC_GLOBAL_BLOCKS);
-    }
+    /* the tail pass pushes particles appended since the last sort; batch
+     * members have none (no moving window, and refresh sorts any tail) */
+    for (int k = 0; k < b->nspecs; k++) b->gblocks[k] = 0;
 
     refresh(b);
     b->ntiles = malloc(sizeof(int) * (size_t)n);
