@@ -118,6 +118,31 @@ def test_fields_within_1e5_after_100_steps(gpu, oracle, cfg):
         o.close()
 
 
+@pytest.mark.parametrize("waves", [0, 2, 8])
+def test_float_atomic_deposit_variants_match_oracle(gpu, oracle, waves):
+    """zpic_set_fixed_deposit(0): the deposit sums the tile windows with
+    float LDS atomics instead of 64-bit fixed point; each variant of that
+    kernel (zpic_set_push_waves) keeps the 100-step parity bars."""
+    cfg = weibel(64, 64, 2, m_q2=-1.0)
+    gpu.zpic_set_fixed_deposit(0)
+    gpu.zpic_set_push_waves(waves)
+    try:
+        g, o = gpu_run(gpu, cfg), oracle_run(oracle, cfg)
+        try:
+            g.step(100)
+            o.step(100)
+            sg, so = g.state(), o.state()
+            for f in ("E", "B", "J"):
+                assert rel_l2(sg[f], so[f]) <= 1e-5, f
+            assert np.allclose(sg["kin_energy"], so["kin_energy"], rtol=1e-5)
+        finally:
+            g.close()
+            o.close()
+    finally:
+        gpu.zpic_set_fixed_deposit(1)
+        gpu.zpic_set_push_waves(0)
+
+
 @pytest.mark.parametrize("case", golden_cases())
 def test_against_reference_golden_vectors(gpu, case):
     """Directly against the reference's stored states: the particle multiset
