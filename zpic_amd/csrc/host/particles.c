@@ -416,7 +416,10 @@ static void ensure_stage(struct zpic_dev_species *d, size_t bytes)
 static void grow_sets(struct zpic_dev_species *d, int need)
 {
     if (need <= d->cap) return;
-    int cap = need + need / 8 + 4096;
+    if (need > ZK_MAX_PARTS)
+        zpic_fatal("species needs %d particle slots; at most %d per species are supported", need, ZK_MAX_PARTS);
+    long long want = (long long)need + need / 8 + 4096;
+    const int cap = want > ZK_MAX_PARTS ? ZK_MAX_PARTS : (int)want;
     zpic_sync();
     for (int k = 0; k < 2; k++) {
         zk_parts old = d->set[k], nw;

@@ -30,6 +30,10 @@ extern "C" {
 /* Cell index that marks a removed particle (absorbing boundary). */
 #define ZK_DEAD (-0x40000000)
 
+/* Most particle slots of one species: the push addresses the particle
+ * arrays with 32-bit byte offsets (4 * index). */
+#define ZK_MAX_PARTS (1 << 30)
+
 /* Per-species counters kept in device memory. */
 typedef struct zk_spec_ctr {
     int np_total;               /* slots in use: tile ranges + unsorted tail */
