@@ -412,8 +412,13 @@ def main() -> int:
     value = npart * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     cells = nx * nx
-    np_species = npart // 2
-    bytes_per_launch = 56.0 * np_species + 36.0 * cells
+    nspec = len(sp)
+    # one push launch covers one species, or all of them when sim_iter fuses
+    # the species (the default): 56 B per particle (read and write the seven
+    # particle fields) + 36 B per cell (read E and B, write J) per launch
+    spl = max(1, round(nspec * args.steps / max(launches.value, 1)))
+    bytes_per_launch = 56.0 * (npart / nspec) * spl + 36.0 * cells
+    launches_per_step = launches.value / args.steps
     avg_push = push_s / max(launches.value, 1)
     achieved = bytes_per_launch / avg_push / 1e9
     traffic = None
@@ -421,14 +426,14 @@ def main() -> int:
     if os.path.exists(tfile):
         try:
             tj = json.load(open(tfile))
-            if tj.get("nx") == nx and tj.get("ppc") == ppc:
+            if tj.get("nx") == nx and tj.get("ppc") == ppc and tj.get("species_per_launch", 1) == spl:
                 traffic = tj.get("bytes_per_launch")
         except Exception:
             traffic = None
 
     # whole step: both species' push bytes + current reset (12 B/cell), two
     # filter passes (24 B/cell each) and the fused Yee update (60 B/cell)
-    step_bytes = 2 * bytes_per_launch + (12.0 + 2 * 24.0 + 60.0) * cells
+    step_bytes = launches_per_step * bytes_per_launch + (12.0 + 2 * 24.0 + 60.0) * cells
     step_achieved = step_bytes / (ms_step * 1e-3) / 1e9
 
     cpu = None
@@ -456,7 +461,10 @@ def main() -> int:
                    "nx": nx, "ppc": ppc, "particles": npart, "steps_total": args.warmup + args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "push_tile_kernel + push_global_kernel (push + deposit, one species)",
+                     "kernel": ("push_species_kernel (push + deposit, all species of the step in one launch)"
+                                if spl > 1 else "push_tile_kernel (push + deposit, one species)")
+                     + " + push_global_kernel",
+                     "species_per_launch": spl,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": 1e3 * avg_push},
         "step_roofline": {"bound": "hbm", "achieved": step_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_achieved / HBM_PEAK_GBS, "bytes_per_step": step_bytes},

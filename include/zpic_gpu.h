@@ -57,6 +57,16 @@ void zpic_set_push_waves(int waves);
  * float LDS atomics. */
 void zpic_set_fixed_deposit(int on);
 
+/* sim_iter pushes all species of a simulation in one tile launch when they
+ * share the tiling and the fixed-point scale (same |q| rounded up to a
+ * power of two) and the fixed-point deposit is on: each tile's fields are
+ * staged and its current window is stored once per step instead of once
+ * per species.  1 (default; environment variable ZPIC_FUSE_SPECIES=0
+ * turns it off) or 0 (one launch per species, as spec_advance).  The
+ * fused window is rounded to FP32 once for all species together, so the
+ * current differs from the per-species launches at FP32 rounding. */
+void zpic_set_species_fusion(int on);
+
 /* Block until all queued GPU work is done. */
 void zpic_synchronize(void);
 

@@ -658,15 +658,13 @@ static void inject_column(t_species *s)
     }
 }
 
-void spec_advance(t_species *s, t_emf *emf, t_current *cur)
+/* Arguments of the push of species s (em2d/particles.c:1104-1117); sets
+ * *move when the window moves this step.  The species, fields and current
+ * must be on the device. */
+static void push_args(t_species *s, t_emf *emf, t_current *cur, zk_push_args *pa, int *move)
 {
-    zpic_species_ensure(s);
     struct zpic_dev_species *d = s->dev;
-    zpic_emf_ensure(emf);
-    zpic_current_ensure(cur, &d->t);
     struct zpic_dev_current *dc = cur->dev;
-
-    zpic_timer_begin(ZPIC_T_SPEC);
     zk_push_args a;
     memset(&a, 0, sizeof(a));
     a.p = d->set[d->cur];
@@ -689,21 +687,23 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     a.qny = s->q * s->dx[1] / s->dt;
     a.q = s->q;
     const int next_iter = s->iter + 1;
-    const int move = s->moving_window && ((next_iter * s->dt) > (s->dx[0] * (s->n_move + 1)));
-    a.shift = move;
+    *move = s->moving_window && ((next_iter * s->dt) > (s->dx[0] * (s->n_move + 1)));
+    a.shift = *move;
     /* the global pass pushes the unsorted tail (injected particles), if any */
-    const int tb = d->tail_upper > 0 ? ZPIC_GLOBAL_BLOCKS : 0;
     a.ovf_list = d->ovf_list;
-    a.global_blocks = tb;
+    a.global_blocks = d->tail_upper > 0 ? ZPIC_GLOBAL_BLOCKS : 0;
     a.waves = zpic_push_waves();
     a.fixed = zpic_fixed_deposit();
     zpic_fixed_point_scale(s->q, &a.fx_scale, &a.fx_unscale);
-    zpic_ktimer_begin(ZPIC_T_PUSH);
-    ZCHECK(zk_push(&a, zpic_stream()));
-    zpic_ktimer_end(ZPIC_T_PUSH);
-    dc->fresh = 0;
+    *pa = a;
+}
 
-    s->iter = next_iter;
+/* The rest of spec_advance after its push: iteration count, window
+ * injection, energy, and the species' sorts. */
+static void after_push(t_species *s, int move, int tb)
+{
+    struct zpic_dev_species *d = s->dev;
+    s->iter++;
     if (move) {
         s->n_move++;
         inject_column(s);
@@ -729,8 +729,61 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
         if (novf * 1000 > (long long)ZPIC_SORT_OVF_PERMILLE * ZPIC_SORT_PROBE * (long long)c.np_total) sort = 1;
     }
     if (sort) sort_now(s, 0);
-    zpic_timer_end(ZPIC_T_SPEC);
     d->host_valid = 0;
+}
+
+void spec_advance(t_species *s, t_emf *emf, t_current *cur)
+{
+    zpic_species_ensure(s);
+    zpic_emf_ensure(emf);
+    zpic_current_ensure(cur, &s->dev->t);
+
+    zpic_timer_begin(ZPIC_T_SPEC);
+    zk_push_args a;
+    int move;
+    push_args(s, emf, cur, &a, &move);
+    zpic_ktimer_begin(ZPIC_T_PUSH);
+    ZCHECK(zk_push(&a, zpic_stream()));
+    zpic_ktimer_end(ZPIC_T_PUSH);
+    cur->dev->fresh = 0;
+    after_push(s, move, a.global_blocks);
+    zpic_timer_end(ZPIC_T_SPEC);
+}
+
+static int same_tiles(const zk_tiles *a, const zk_tiles *b)
+{
+    return a->nx == b->nx && a->ny == b->ny && a->TX == b->TX && a->TY == b->TY && a->H == b->H &&
+           a->WX == b->WX && a->WY == b->WY && a->ntx == b->ntx && a->nty == b->nty && a->ntiles == b->ntiles;
+}
+
+void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur)
+{
+    int fuse = n >= 2 && n <= ZK_FUSE_MAX && zpic_species_fusion() && zpic_fixed_deposit();
+    if (fuse) {
+        zpic_emf_ensure(emf);
+        for (int k = 0; k < n; k++) {
+            zpic_species_ensure(&spec[k]);
+            zpic_current_ensure(cur, &spec[k].dev->t);
+        }
+        zk_push_multi m;
+        int move[ZK_FUSE_MAX];
+        memset(&m, 0, sizeof(m));
+        m.n = n;
+        for (int k = 0; k < n; k++) push_args(&spec[k], emf, cur, &m.s[k], &move[k]);
+        for (int k = 1; k < n && fuse; k++)
+            fuse = same_tiles(&m.s[k].t, &m.s[0].t) && m.s[k].fx_scale == m.s[0].fx_scale;
+        if (fuse) {
+            zpic_timer_begin(ZPIC_T_SPEC);
+            zpic_ktimer_begin(ZPIC_T_PUSH);
+            ZCHECK(zk_push_species(&m, zpic_stream()));
+            zpic_ktimer_end(ZPIC_T_PUSH);
+            cur->dev->fresh = 0;
+            for (int k = 0; k < n; k++) after_push(&spec[k], move[k], m.s[k].global_blocks);
+            zpic_timer_end(ZPIC_T_SPEC);
+            return;
+        }
+    }
+    for (int k = 0; k < n; k++) spec_advance(&spec[k], emf, cur);
 }
 
 void spec_move_window(t_species *s)

@@ -40,6 +40,18 @@ int zpic_fixed_deposit(void)
     return g_fixed_deposit;
 }
 
+/* -1: not chosen yet (environment variable ZPIC_FUSE_SPECIES, default on) */
+static int g_fuse_species = -1;
+void zpic_set_species_fusion(int on) { g_fuse_species = on ? 1 : 0; }
+int zpic_species_fusion(void)
+{
+    if (g_fuse_species < 0) {
+        const char *e = getenv("ZPIC_FUSE_SPECIES");
+        g_fuse_species = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_fuse_species;
+}
+
 void zpic_fixed_point_scale(float q, float *scale, float *unscale)
 {
     /* Q = 2^k, the power of two at or above |q| */

@@ -100,6 +100,9 @@ struct zpic_dev_emf {
 
 /* helpers shared by the host modules */
 void zpic_species_ensure(t_species *spec);
+/* spec_advance of n species of one simulation, in one fused tile launch
+ * when they share tiling and fixed-point scale (zpic_set_species_fusion) */
+void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur);
 void zpic_emf_ensure(t_emf *emf);
 void zpic_current_ensure(t_current *cur, const zk_tiles *t);
 const float *zpic_emf_particle_E(t_emf *emf);

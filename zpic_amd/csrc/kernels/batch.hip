@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_batch(const zk_bspec
 {
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     const zk_push_args a = spec_args(S, sims[S.sim]);
-    push_tile_body<1, FX>(a, items[2 * blockIdx.x + 1]);
+    push_tile_body<1, FX>(OneSpecies{a}, items[2 * blockIdx.x + 1]);
 }
 
 __global__ void __launch_bounds__(256) push_global_batch(const zk_bspec *specs, const zk_bsim *sims,
@@ -178,7 +178,7 @@ extern "C" int zk_batch_push(const zk_bspec *specs, const zk_bsim *sims, const i
 {
     if (ntile > 0) {
         if (fixed)
-            hipLaunchKernelGGL((push_tile_batch<1, true>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
+            hipLaunchKernelGGL((push_tile_batch<5, true>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
         else if (waves == 8)
             hipLaunchKernelGGL((push_tile_batch<8, false>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
         else
@@ -192,7 +192,7 @@ extern "C" int zk_batch_prepare_push(size_t lds)
 {
     if (lds <= 65536) return 0;
     const void *fns[3] = {(const void *)push_tile_batch<1, false>, (const void *)push_tile_batch<8, false>,
-                          (const void *)push_tile_batch<1, true>};
+                          (const void *)push_tile_batch<5, true>};
     for (int k = 0; k < 3; k++) {
         const int e = (int)hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e) return e;

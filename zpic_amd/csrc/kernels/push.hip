@@ -34,7 +34,13 @@ namespace {
 template <int MIN_WAVES, int PAIR, bool FX>
 __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args a)
 {
-    push_tile_body<PAIR, FX>(a, zk_xcd_item(blockIdx.x, a.t.ntiles));
+    push_tile_body<PAIR, FX>(OneSpecies{a}, zk_xcd_item(blockIdx.x, a.t.ntiles));
+}
+
+/* every species of a simulation in one launch (shared staging and window) */
+__global__ void __launch_bounds__(256, 5) push_species_kernel(zk_push_multi m)
+{
+    push_tile_body<1, true>(SpeciesSet{m}, zk_xcd_item(blockIdx.x, m.s[0].t.ntiles));
 }
 
 __global__ void __launch_bounds__(256) push_global_kernel(zk_push_args a)
@@ -65,7 +71,7 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
     /* variants: 0 lets the compiler keep every value in registers (about
      * 6 waves per SIMD); 8 asks for 8 waves per SIMD at the price of a few
      * register spills; 2 pushes two particles per thread and iteration */
-    const void *fn = a->fixed         ? (const void *)push_tile_kernel<1, 1, true>
+    const void *fn = a->fixed         ? (const void *)push_tile_kernel<5, 1, true>
                      : a->waves == 8 ? (const void *)push_tile_kernel<8, 1, false>
                      : a->waves == 2 ? (const void *)push_tile_kernel<1, 2, false>
                                      : (const void *)push_tile_kernel<1, 1, false>;
@@ -76,7 +82,7 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
     }
     if (a->t.ntiles > 0) {
         if (a->fixed)
-            hipLaunchKernelGGL((push_tile_kernel<1, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<5, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else if (a->waves == 8)
             hipLaunchKernelGGL((push_tile_kernel<8, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else if (a->waves == 2)
@@ -85,6 +91,24 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
             hipLaunchKernelGGL((push_tile_kernel<1, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
     }
     if (a->global_blocks > 0) hipLaunchKernelGGL(push_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_push_species(const zk_push_multi *m, zk_stream_t s)
+{
+    const zk_push_args *a = &m->s[0];
+    if (m->n < 1 || m->n > ZK_FUSE_MAX || !a->fixed) return (int)hipErrorInvalidValue;
+    const int n = a->t.WX * a->t.WY;
+    const size_t lds = (size_t)6 * n * sizeof(float) + (size_t)3 * n * 8;
+    if (lds > 65536) {
+        const int e = (int)hipFuncSetAttribute((const void *)push_species_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e) return e;
+    }
+    if (a->t.ntiles > 0) hipLaunchKernelGGL(push_species_kernel, dim3(a->t.ntiles), dim3(256), lds, s, *m);
+    for (int k = 0; k < m->n; k++)
+        if (m->s[k].global_blocks > 0)
+            hipLaunchKernelGGL(push_global_kernel, dim3(m->s[k].global_blocks), dim3(256), 0, s, m->s[k]);
     ZK_LAUNCH_CHECK();
 }
 

@@ -337,21 +337,37 @@ __device__ inline void list_out(const zk_push_args &a, int i0, int i, bool out_h
     }
 }
 
-/* Tile pass.  PAIR = 2 pushes two particles per thread and iteration: the
+/* The species a tile workgroup pushes: one (the plain and batched
+ * launches), or several species of one simulation that share the tiling,
+ * the fields, the current and the fixed-point scale (fused launch). */
+struct OneSpecies {
+    const zk_push_args &a;
+    __device__ int n() const { return 1; }
+    __device__ const zk_push_args &operator[](int) const { return a; }
+};
+
+struct SpeciesSet {
+    const zk_push_multi &m;
+    __device__ int n() const { return m.n; }
+    __device__ const zk_push_args &operator[](int k) const { return m.s[k]; }
+};
+
+/* Tile pass.  The workgroup stages the fields of its window and clears
+ * the current window once, pushes the tile's particles of each species in
+ * turn, and stores the window once: with several species the staging and
+ * the window store (0.12 ms of a 2048^2 launch) are not repeated per
+ * species.  PAIR = 2 pushes two particles per thread and iteration: the
  * gathers and Boris pushes of both are independent and interleave, which
  * hides LDS and arithmetic latency at the price of registers.  FX selects
  * the fixed-point window (LdsCurrentFx) instead of float LDS atomics. */
-template <int PAIR = 1, bool FX = false>
-__device__ inline void push_tile_body(const zk_push_args &a, const int tile)
+template <int PAIR = 1, bool FX = false, class Set>
+__device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
 {
     extern __shared__ float sm[];
     __shared__ double red[16];
     __shared__ int ndead_blk, nlist;
-    const zk_tiles &t = a.t;
-    const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
-    zk_parts P = a.p;
-    double kin = 0.0;
-    int ndead = 0;
+    const zk_push_args &a0 = S[0];
+    const zk_tiles &t = a0.t;
     if (threadIdx.x == 0) ndead_blk = nlist = 0;
 
     const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
@@ -365,13 +381,13 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     for (int m = threadIdx.x; m < n; m += blockDim.x) {
         const int gx = ox + m % t.WX, gy = oy + m / t.WX;
         const bool in = gx >= -1 && gx <= t.nx + 1 && gy >= -1 && gy <= t.ny + 1;
-        const int g = 3 * ((gx + 1) + (gy + 1) * a.nrow);
-        fl[0 * n + m] = in ? a.E[g + 0] : 0.0f;
-        fl[1 * n + m] = in ? a.E[g + 1] : 0.0f;
-        fl[2 * n + m] = in ? a.E[g + 2] : 0.0f;
-        fl[3 * n + m] = in ? a.B[g + 0] : 0.0f;
-        fl[4 * n + m] = in ? a.B[g + 1] : 0.0f;
-        fl[5 * n + m] = in ? a.B[g + 2] : 0.0f;
+        const int g = 3 * ((gx + 1) + (gy + 1) * a0.nrow);
+        fl[0 * n + m] = in ? a0.E[g + 0] : 0.0f;
+        fl[1 * n + m] = in ? a0.E[g + 1] : 0.0f;
+        fl[2 * n + m] = in ? a0.E[g + 2] : 0.0f;
+        fl[3 * n + m] = in ? a0.B[g + 0] : 0.0f;
+        fl[4 * n + m] = in ? a0.B[g + 1] : 0.0f;
+        fl[5 * n + m] = in ? a0.B[g + 2] : 0.0f;
         if (FX) {
             jx[0 * n + m] = 0ull;
             jx[1 * n + m] = 0ull;
@@ -390,129 +406,140 @@ __device__ inline void push_tile_body(const zk_push_args &a, const int tile)
     lj.n = n;
     if constexpr (FX) {
         lj.j = jx;
-        lj.scale = a.fx_scale;
+        lj.scale = a0.fx_scale;
     } else {
         lj.j = jl;
     }
-    const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
-    if (PAIR == 1) {
-        /* Each thread pushes particles i, i + bs, i + 2 bs, ... with the
-         * loads running two particles ahead in three register sets that
-         * take turns (A, B, C), so that the loads' latency is hidden behind
-         * two pushes.  The sets are unrolled rather than copied at the
-         * loop's back edge: such a copy waits for every outstanding memory
-         * operation, the previous particle's stores included.  Loads past
-         * the range re-read its last particle and are not pushed.  (2048^2
-         * Weibel push: 1.16 ms per launch with no load ahead, 1.08 ms with
-         * one particle ahead, 1.03-1.05 ms with two; three ahead spills.) */
-        auto step = [&](Part &c, int i) {
-            const int lx = c.ix - tx0, ly = c.iy - ty0;
-            const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
-            const bool live = i < i1 && c.ix != ZK_DEAD;
-            list_out(a, i0, i, live && !inside, &nlist);
-            if (live && inside) {
+    for (int sp = 0; sp < S.n(); sp++) {
+        const zk_push_args &a = S[sp];
+        const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
+        zk_parts P = a.p;
+        double kin = 0.0;
+        int ndead = 0;
+        const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
+        if (PAIR == 1) {
+            /* Each thread pushes particles i, i + bs, i + 2 bs, ... with the
+             * loads running two particles ahead in three register sets that
+             * take turns (A, B, C), so that the loads' latency is hidden behind
+             * two pushes.  The sets are unrolled rather than copied at the
+             * loop's back edge: such a copy waits for every outstanding memory
+             * operation, the previous particle's stores included.  Loads past
+             * the range re-read its last particle and are not pushed.  (2048^2
+             * Weibel push: 1.16 ms per launch with no load ahead, 1.08 ms with
+             * one particle ahead, 1.03-1.05 ms with two; three ahead spills.) */
+            auto step = [&](Part &c, int i) {
+                const int lx = c.ix - tx0, ly = c.iy - ty0;
+                const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
+                const bool live = i < i1 && c.ix != ZK_DEAD;
+                list_out(a, i0, i, live && !inside, &nlist);
+                if (live && inside) {
+                    int di, dj;
+                    kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+                    c.ix += di;
+                    c.iy += dj;
+                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    store_part(P, i, c);
+                }
+            };
+            const int bs = blockDim.x, last = i1 - 1;
+            int i = i0 + threadIdx.x;
+            Part ca, cb, cc;
+            if (i < i1) {
+                ca = load_part(P, i);
+                cb = load_part(P, min(i + bs, last));
+            }
+            for (; i < i1; i += 3 * bs) {
+                cc = load_part(P, min(i + 2 * bs, last));
+                step(ca, i);
+                ca = load_part(P, min(i + 3 * bs, last));
+                step(cb, i + bs);
+                cb = load_part(P, min(i + 4 * bs, last));
+                step(cc, i + 2 * bs);
+            }
+        } else {
+            for (int i = i0 + threadIdx.x; i < i1; i += 2 * blockDim.x) {
+                const int j = i + blockDim.x;
+                const bool hasb = j < i1;
+                Part ca = load_part(P, i);
+                Part cb = hasb ? load_part(P, j) : ca;
+                const int lxa = ca.ix - tx0, lya = ca.iy - ty0, lxb = cb.ix - tx0, lyb = cb.iy - ty0;
+                const bool ina = lxa >= -t.H && lxa < t.TX + t.H && lya >= -t.H && lya < t.TY + t.H;
+                const bool inb = lxb >= -t.H && lxb < t.TX + t.H && lyb >= -t.H && lyb < t.TY + t.H;
+                const bool livea = ca.ix != ZK_DEAD, liveb = hasb && cb.ix != ZK_DEAD;
+                list_out(a, i0, i, livea && !ina, &nlist);
+                list_out(a, i0, j, liveb && !inb, &nlist);
+                const bool pa = livea && ina, pb = liveb && inb;
+                /* a particle that is not pushed here moves at a safe cell */
+                const int cia = pa ? ca.ix - ox : t.H + 1, cja = pa ? ca.iy - oy : t.H + 1;
+                const int cib = pb ? cb.ix - ox : t.H + 1, cjb = pb ? cb.iy - oy : t.H + 1;
+                float mxa, mya, qa, mxb, myb, qb;
+                int dia, dja, dib, djb;
+                const double ka = move_one(lf, cia, cja, ca.x, ca.y, ca.ux, ca.uy, ca.uz, mxa, mya, qa, dia, dja, k);
+                const double kb = move_one(lf, cib, cjb, cb.x, cb.y, cb.ux, cb.uy, cb.uz, mxb, myb, qb, dib, djb, k);
+                if (pa) {
+                    float x1 = ca.x + mxa, y1 = ca.y + mya;
+                    x1 -= dia;
+                    y1 -= dja;
+                    deposit(lj, cia, cja, dia, dja, ca.x, ca.y, mxa, mya, k.qnx, k.qny, qa);
+                    ca.x = x1;
+                    ca.y = y1;
+                    kin += ka;
+                    ca.ix += dia;
+                    ca.iy += dja;
+                    if (boundaries(ca.ix, ca.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    store_part(P, i, ca);
+                }
+                if (pb) {
+                    float x1 = cb.x + mxb, y1 = cb.y + myb;
+                    x1 -= dib;
+                    y1 -= djb;
+                    deposit(lj, cib, cjb, dib, djb, cb.x, cb.y, mxb, myb, k.qnx, k.qny, qb);
+                    cb.x = x1;
+                    cb.y = y1;
+                    kin += kb;
+                    cb.ix += dib;
+                    cb.iy += djb;
+                    if (boundaries(cb.ix, cb.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    store_part(P, j, cb);
+                }
+            }
+        }
+        __syncthreads();
+        /* the tile's particles outside its window: global path, by this block
+         * (gathers from the guard-inclusive grids, atomics into Jov) */
+        const int nl = nlist;
+        if (nl > 0) {
+            const GlobalFields gf = {a.E, a.B, a.nrow};
+            const GlobalCurrent gj = {a.Jov, a.nrow};
+            for (int q = threadIdx.x; q < nl; q += blockDim.x) {
+                const int i = a.ovf_list[i0 + q];
+                Part c = load_part(P, i);
                 int di, dj;
-                kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+                kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
                 c.ix += di;
                 c.iy += dj;
                 if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
                 store_part(P, i, c);
             }
-        };
-        const int bs = blockDim.x, last = i1 - 1;
-        int i = i0 + threadIdx.x;
-        Part ca, cb, cc;
-        if (i < i1) {
-            ca = load_part(P, i);
-            cb = load_part(P, min(i + bs, last));
+            if (threadIdx.x == 0) atomicAdd(&a.ctr->n_overflow, nl);
         }
-        for (; i < i1; i += 3 * bs) {
-            cc = load_part(P, min(i + 2 * bs, last));
-            step(ca, i);
-            ca = load_part(P, min(i + 3 * bs, last));
-            step(cb, i + bs);
-            cb = load_part(P, min(i + 4 * bs, last));
-            step(cc, i + 2 * bs);
+        /* per-tile energy partial (summed in fixed order by zk_spec_finalize) */
+        const double e = zk_block_sum(kin, red);
+        if (threadIdx.x == 0) a.epart[tile] = e;
+        if (ndead) atomicAdd(&ndead_blk, ndead);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
+            ndead_blk = nlist = 0;
         }
-    } else {
-        for (int i = i0 + threadIdx.x; i < i1; i += 2 * blockDim.x) {
-            const int j = i + blockDim.x;
-            const bool hasb = j < i1;
-            Part ca = load_part(P, i);
-            Part cb = hasb ? load_part(P, j) : ca;
-            const int lxa = ca.ix - tx0, lya = ca.iy - ty0, lxb = cb.ix - tx0, lyb = cb.iy - ty0;
-            const bool ina = lxa >= -t.H && lxa < t.TX + t.H && lya >= -t.H && lya < t.TY + t.H;
-            const bool inb = lxb >= -t.H && lxb < t.TX + t.H && lyb >= -t.H && lyb < t.TY + t.H;
-            const bool livea = ca.ix != ZK_DEAD, liveb = hasb && cb.ix != ZK_DEAD;
-            list_out(a, i0, i, livea && !ina, &nlist);
-            list_out(a, i0, j, liveb && !inb, &nlist);
-            const bool pa = livea && ina, pb = liveb && inb;
-            /* a particle that is not pushed here moves at a safe cell */
-            const int cia = pa ? ca.ix - ox : t.H + 1, cja = pa ? ca.iy - oy : t.H + 1;
-            const int cib = pb ? cb.ix - ox : t.H + 1, cjb = pb ? cb.iy - oy : t.H + 1;
-            float mxa, mya, qa, mxb, myb, qb;
-            int dia, dja, dib, djb;
-            const double ka = move_one(lf, cia, cja, ca.x, ca.y, ca.ux, ca.uy, ca.uz, mxa, mya, qa, dia, dja, k);
-            const double kb = move_one(lf, cib, cjb, cb.x, cb.y, cb.ux, cb.uy, cb.uz, mxb, myb, qb, dib, djb, k);
-            if (pa) {
-                float x1 = ca.x + mxa, y1 = ca.y + mya;
-                x1 -= dia;
-                y1 -= dja;
-                deposit(lj, cia, cja, dia, dja, ca.x, ca.y, mxa, mya, k.qnx, k.qny, qa);
-                ca.x = x1;
-                ca.y = y1;
-                kin += ka;
-                ca.ix += dia;
-                ca.iy += dja;
-                if (boundaries(ca.ix, ca.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-                store_part(P, i, ca);
-            }
-            if (pb) {
-                float x1 = cb.x + mxb, y1 = cb.y + myb;
-                x1 -= dib;
-                y1 -= djb;
-                deposit(lj, cib, cjb, dib, djb, cb.x, cb.y, mxb, myb, k.qnx, k.qny, qb);
-                cb.x = x1;
-                cb.y = y1;
-                kin += kb;
-                cb.ix += dib;
-                cb.iy += djb;
-                if (boundaries(cb.ix, cb.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-                store_part(P, j, cb);
-            }
-        }
-    }
-    __syncthreads();
-    /* the tile's particles outside its window: global path, by this block
-     * (gathers from the guard-inclusive grids, atomics into Jov) */
-    const int nl = nlist;
-    if (nl > 0) {
-        const GlobalFields gf = {a.E, a.B, a.nrow};
-        const GlobalCurrent gj = {a.Jov, a.nrow};
-        for (int q = threadIdx.x; q < nl; q += blockDim.x) {
-            const int i = a.ovf_list[i0 + q];
-            Part c = load_part(P, i);
-            int di, dj;
-            kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-            c.ix += di;
-            c.iy += dj;
-            if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
-            store_part(P, i, c);
-        }
-        if (threadIdx.x == 0) atomicAdd(&a.ctr->n_overflow, nl);
+        __syncthreads();
     }
     /* store the current window of this tile */
-    float *dst = a.region + (size_t)tile * 3 * n;
+    float *dst = a0.region + (size_t)tile * 3 * n;
     for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) {
-        const float v = FX ? (float)(long long)jx[m] * a.fx_unscale : jl[m];
-        dst[m] = a.fresh ? v : dst[m] + v;
+        const float v = FX ? (float)(long long)jx[m] * a0.fx_unscale : jl[m];
+        dst[m] = a0.fresh ? v : dst[m] + v;
     }
-    /* per-tile energy partial (summed in fixed order by zk_spec_finalize) */
-    const double s = zk_block_sum(kin, red);
-    if (threadIdx.x == 0) a.epart[tile] = s;
-    if (ndead) atomicAdd(&ndead_blk, ndead);
-    __syncthreads();
-    if (threadIdx.x == 0 && ndead_blk) atomicAdd(&a.ctr->n_dead, ndead_blk);
 }
 
 /* Global pass: the unsorted tail (particles injected since the last sort),

@@ -82,8 +82,19 @@ typedef struct zk_push_args {
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
 
+/* Species of one simulation pushed by one tile launch: same tiling,
+ * fields, current windows and fixed-point scale; s[0] supplies those. */
+#define ZK_FUSE_MAX 4
+typedef struct zk_push_multi {
+    zk_push_args s[ZK_FUSE_MAX];
+    int n;
+} zk_push_multi;
+
 int zk_fill_f32(float *p, size_t n, float v, zk_stream_t s);
 int zk_push(const zk_push_args *a, zk_stream_t s);
+/* fused tile pass over m->n species (fixed-point windows), then the
+ * global pass of each species that has an unsorted tail */
+int zk_push_species(const zk_push_multi *m, zk_stream_t s);
 int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int nparts,
                             double qm, double dx0, double dx1, zk_stream_t s);
 
