@@ -11,6 +11,17 @@
 
 #define ZK_WAVE 64
 
+/* Element at byte offset o of an array.  The particle arrays of the push
+ * and of the re-sort are addressed as a uniform base plus one shared 32-bit
+ * offset (the global saddr form): one offset register instead of a 64-bit
+ * address per array, which leaves registers for more loads in flight.
+ * Offsets fit 32 bits because a species holds at most ZK_MAX_PARTS slots. */
+template <class T>
+__device__ inline T &zk_at(T *base, unsigned o)
+{
+    return *(T *)((char *)base + o);
+}
+
 /* Blocks are dealt to the 8 XCDs round-robin.  Map block b to a work item
  * so that each XCD receives one contiguous run of items: neighbouring tiles
  * then share the halo lines in that XCD's L2. */

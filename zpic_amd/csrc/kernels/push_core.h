@@ -203,17 +203,6 @@ struct Part {
     float x, y, ux, uy, uz;
 };
 
-/* Element at byte offset o of an array.  The seven particle arrays are
- * addressed as a uniform base plus one shared 32-bit offset (the global
- * saddr form): one offset register instead of seven 64-bit addresses,
- * which leaves room for the loads ahead in the tile pass.  Offsets fit
- * 32 bits because a species holds at most ZK_MAX_PARTS slots. */
-template <class T>
-__device__ inline T &at(T *base, unsigned o)
-{
-    return *(T *)((char *)base + o);
-}
-
 /* Particle streams are read and written once per step with ordinary
  * loads and stores: nontemporal hints on them made the 2048^2 Weibel push
  * 14 % slower (1.35 against 1.16 ms per launch). */
@@ -221,26 +210,26 @@ __device__ inline Part load_part(const zk_parts &P, int i)
 {
     const unsigned o = (unsigned)i * 4u;
     Part c;
-    c.ix = at(P.ix, o);
-    c.iy = at(P.iy, o);
-    c.x = at(P.x, o);
-    c.y = at(P.y, o);
-    c.ux = at(P.ux, o);
-    c.uy = at(P.uy, o);
-    c.uz = at(P.uz, o);
+    c.ix = zk_at(P.ix, o);
+    c.iy = zk_at(P.iy, o);
+    c.x = zk_at(P.x, o);
+    c.y = zk_at(P.y, o);
+    c.ux = zk_at(P.ux, o);
+    c.uy = zk_at(P.uy, o);
+    c.uz = zk_at(P.uz, o);
     return c;
 }
 
 __device__ inline void store_part(const zk_parts &P, int i, const Part &c)
 {
     const unsigned o = (unsigned)i * 4u;
-    at(P.ix, o) = c.ix;
-    at(P.iy, o) = c.iy;
-    at(P.x, o) = c.x;
-    at(P.y, o) = c.y;
-    at(P.ux, o) = c.ux;
-    at(P.uy, o) = c.uy;
-    at(P.uz, o) = c.uz;
+    zk_at(P.ix, o) = c.ix;
+    zk_at(P.iy, o) = c.iy;
+    zk_at(P.x, o) = c.x;
+    zk_at(P.y, o) = c.y;
+    zk_at(P.ux, o) = c.ux;
+    zk_at(P.uy, o) = c.uy;
+    zk_at(P.uz, o) = c.uz;
 }
 
 /* Motion of one particle whose cell is (ci, cj) in the sources' index

@@ -134,8 +134,10 @@ __device__ inline void scatter_body(const zk_parts &src, const zk_parts &dst, co
  * 256-thread block */
 #define ZK_NBR 13
 #define ZK_SORT_THREADS 256
-/* 64-particle chunks a wave loads before processing them (loads in flight) */
-#define ZK_SORT_UNROLL 4
+/* 64-particle chunks a wave loads before processing them (loads in flight;
+ * 2048^2 Weibel re-sort: 1.45 ms with 4, 1.35 ms with 8, 1.31 ms with 12
+ * at occupancy 3, 1.36 ms with 16) */
+#define ZK_SORT_UNROLL 8
 
 /* piece [*lo, *hi) of the tile range [a, b) owned by wave w of nw */
 __device__ inline void wave_piece(int a, int b, int w, int nw, int *lo, int *hi)
@@ -333,13 +335,14 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
             for (int u = 0; u < ZK_SORT_UNROLL; u++) {
                 const int i = base + u * ZK_WAVE + lane;
                 const bool in = i < whi;
-                cx[u] = in ? src.ix[i] : ZK_DEAD;
-                cy[u] = in ? src.iy[i] : 0;
-                x[u] = in ? src.x[i] : 0.f;
-                y[u] = in ? src.y[i] : 0.f;
-                ux[u] = in ? src.ux[i] : 0.f;
-                uy[u] = in ? src.uy[i] : 0.f;
-                uz[u] = in ? src.uz[i] : 0.f;
+                const unsigned o = (unsigned)i * 4u;
+                cx[u] = in ? zk_at(src.ix, o) : ZK_DEAD;
+                cy[u] = in ? zk_at(src.iy, o) : 0;
+                x[u] = in ? zk_at(src.x, o) : 0.f;
+                y[u] = in ? zk_at(src.y, o) : 0.f;
+                ux[u] = in ? zk_at(src.ux, o) : 0.f;
+                uy[u] = in ? zk_at(src.uy, o) : 0.f;
+                uz[u] = in ? zk_at(src.uz, o) : 0.f;
                 id[u] = (in && src.id) ? src.id[i] : 0;
             }
 #pragma unroll
@@ -356,13 +359,14 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
                 const int dm = zk_run_reserve(far ? key : -1, cursor);
                 if (far) d = dm;
                 if (key >= 0) {
-                    dst.ix[d] = cx[u];
-                    dst.iy[d] = cy[u];
-                    dst.x[d] = x[u];
-                    dst.y[d] = y[u];
-                    dst.ux[d] = ux[u];
-                    dst.uy[d] = uy[u];
-                    dst.uz[d] = uz[u];
+                    const unsigned od = (unsigned)d * 4u;
+                    zk_at(dst.ix, od) = cx[u];
+                    zk_at(dst.iy, od) = cy[u];
+                    zk_at(dst.x, od) = x[u];
+                    zk_at(dst.y, od) = y[u];
+                    zk_at(dst.ux, od) = ux[u];
+                    zk_at(dst.uy, od) = uy[u];
+                    zk_at(dst.uz, od) = uz[u];
                     if (src.id) dst.id[d] = id[u];
                 }
             }
