@@ -51,12 +51,24 @@ def matched(gpu, oracle, g, o, k):
     return pg, po
 
 
+def lwfa_with_ions(**kw):
+    """LWFA deck plus an ion species of the same density and particles per
+    cell: the charges have the same magnitude, so sim_iter pushes both
+    species in one launch, with the window shift, the injected columns and
+    the unsorted tail of each."""
+    cfg = lwfa(**kw)
+    ions = dict(cfg["species"][0], name="ions", m_q=1836.0)
+    cfg["species"] = [cfg["species"][0], ions]
+    return cfg
+
+
 CASES = {
     "weibel": weibel(32, 32, 2),
     "weibel_2e_4ppc": weibel(48, 32, 4, n_sort=20, m_q2=-1.0),
     "smoothed": weibel(64, 48, 2, n_sort=8, smooth={"xtype": "compensated", "ytype": "binomial",
                                                     "xlevel": 2, "ylevel": 1}),
     "lwfa_moving_window": lwfa(200, 32, start=6.0),
+    "lwfa_electrons_ions": lwfa_with_ions(nx=200, ny=32, start=6.0),
     "hot_fast": weibel(32, 32, 3, uz=0.9, uth=0.4, n_sort=5),
 }
 
