@@ -207,8 +207,8 @@ def adjoint_workload(args) -> dict:
     from zpic_amd.diff import WeibelConfig, WeibelRun
 
     lib = _lib.load()
-    if args.tile or args.halo:
-        lib.zpic_set_tiling(args.tile, args.tile, args.halo)
+    if args.tile or args.tile_y or args.halo:
+        lib.zpic_set_tiling(args.tile, args.tile_y or args.tile, args.halo)
     cfg = WeibelConfig()
     seed = 1e-3 * np.random.default_rng(0).standard_normal((cfg.ny // cfg.block, cfg.nx // cfg.block))
     # warm-up run (kernel loading, allocations)
@@ -323,6 +323,7 @@ def main() -> int:
     ap.add_argument("--ppc", type=int, default=4)
     ap.add_argument("--n-sort", type=int, default=20)
     ap.add_argument("--tile", type=int, default=0, help="tile width in cells (0: library default)")
+    ap.add_argument("--tile-y", type=int, default=0, help="tile height in cells (0: the tile width)")
     ap.add_argument("--halo", type=int, default=0, help="tile drift margin in cells (0: library default)")
     ap.add_argument("--waves", type=int, default=0, help="push variant: 0 (default), 2 or 8")
     ap.add_argument("--fixed", type=int, default=-1, help="fixed-point current deposit: 1 on, 0 off (-1: library default)")
@@ -367,8 +368,8 @@ def main() -> int:
     from zpic_amd import em2d
 
     lib = _lib.load()
-    if args.tile or args.halo:
-        lib.zpic_set_tiling(args.tile, args.tile, args.halo)
+    if args.tile or args.tile_y or args.halo:
+        lib.zpic_set_tiling(args.tile, args.tile_y or args.tile, args.halo)
     lib.zpic_set_push_waves(args.waves)
     if args.fixed >= 0:
         lib.zpic_set_fixed_deposit(args.fixed)
