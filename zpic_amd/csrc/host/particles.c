@@ -699,8 +699,9 @@ static void push_args(t_species *s, t_emf *emf, t_current *cur, zk_push_args *pa
 }
 
 /* The rest of spec_advance after its push: iteration count, window
- * injection, energy, and the species' sorts. */
-static void after_push(t_species *s, int move, int tb)
+ * injection and energy (push_done), then the decision whether the tiles
+ * are sorted now (sort_due). */
+static void push_done(t_species *s, int move, int tb)
 {
     struct zpic_dev_species *d = s->dev;
     s->iter++;
@@ -713,6 +714,12 @@ static void after_push(t_species *s, int move, int tb)
     ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0],
                             (double)s->dx[1],
                             zpic_stream()));
+    d->host_valid = 0;
+}
+
+static int sort_due(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
     /* the species' schedule (em2d/particles.c:1262-1264); particles that
      * left their tile windows take the slow global path, so the tiles are
      * also re-sorted early when that path carries too many pushes (the
@@ -728,8 +735,7 @@ static void after_push(t_species *s, int move, int tb)
         d->ovf_seen = (long long)(unsigned)c.n_overflow;
         if (novf * 1000 > (long long)ZPIC_SORT_OVF_PERMILLE * ZPIC_SORT_PROBE * (long long)c.np_total) sort = 1;
     }
-    if (sort) sort_now(s, 0);
-    d->host_valid = 0;
+    return sort;
 }
 
 void spec_advance(t_species *s, t_emf *emf, t_current *cur)
@@ -746,7 +752,8 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     ZCHECK(zk_push(&a, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_PUSH);
     cur->dev->fresh = 0;
-    after_push(s, move, a.global_blocks);
+    push_done(s, move, a.global_blocks);
+    if (sort_due(s)) sort_now(s, 0);
     zpic_timer_end(ZPIC_T_SPEC);
 }
 
@@ -778,7 +785,14 @@ void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur)
             ZCHECK(zk_push_species(&m, zpic_stream()));
             zpic_ktimer_end(ZPIC_T_PUSH);
             cur->dev->fresh = 0;
-            for (int k = 0; k < n; k++) after_push(&spec[k], move[k], m.s[k].global_blocks);
+            /* every species' energy is queued before the sort decisions,
+             * and the decisions (whose counter reads wait for the queue)
+             * are taken before any sort is queued */
+            int due[ZK_FUSE_MAX];
+            for (int k = 0; k < n; k++) push_done(&spec[k], move[k], m.s[k].global_blocks);
+            for (int k = 0; k < n; k++) due[k] = sort_due(&spec[k]);
+            for (int k = 0; k < n; k++)
+                if (due[k]) sort_now(&spec[k], 0);
             zpic_timer_end(ZPIC_T_SPEC);
             return;
         }
