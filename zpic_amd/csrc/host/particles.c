@@ -699,9 +699,10 @@ static void push_args(t_species *s, t_emf *emf, t_current *cur, zk_push_args *pa
 }
 
 /* The rest of spec_advance after its push: iteration count, window
- * injection and energy (push_done), then the decision whether the tiles
- * are sorted now (sort_due). */
-static void push_done(t_species *s, int move, int tb)
+ * injection and energy (push_done; with fin set the energy is left to
+ * the caller's zk_spec_finalize_set, which gets this species' entry),
+ * then the decision whether the tiles are sorted now (sort_due). */
+static void push_done(t_species *s, int move, int tb, zk_finalize_set *fin)
 {
     struct zpic_dev_species *d = s->dev;
     s->iter++;
@@ -711,9 +712,18 @@ static void push_done(t_species *s, int move, int tb)
         d->moved_since_sort++;
     }
     const float qm = s->q * s->m_q;
-    ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0],
-                            (double)s->dx[1],
-                            zpic_stream()));
+    if (fin) {
+        const int k = fin->n++;
+        fin->ctr[k] = d->ctr;
+        fin->epart[k] = d->epart;
+        fin->nparts[k] = d->t.ntiles + tb;
+        fin->qm[k] = (double)qm;
+        fin->dx0[k] = (double)s->dx[0];
+        fin->dx1[k] = (double)s->dx[1];
+    } else {
+        ZCHECK(zk_spec_finalize(d->ctr, d->epart, d->t.ntiles + tb, (double)qm, (double)s->dx[0],
+                                (double)s->dx[1], zpic_stream()));
+    }
     d->host_valid = 0;
 }
 
@@ -752,7 +762,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     ZCHECK(zk_push(&a, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_PUSH);
     cur->dev->fresh = 0;
-    push_done(s, move, a.global_blocks);
+    push_done(s, move, a.global_blocks, NULL);
     if (sort_due(s)) sort_now(s, 0);
     zpic_timer_end(ZPIC_T_SPEC);
 }
@@ -789,7 +799,10 @@ void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur)
              * and the decisions (whose counter reads wait for the queue)
              * are taken before any sort is queued */
             int due[ZK_FUSE_MAX];
-            for (int k = 0; k < n; k++) push_done(&spec[k], move[k], m.s[k].global_blocks);
+            zk_finalize_set fin;
+            memset(&fin, 0, sizeof(fin));
+            for (int k = 0; k < n; k++) push_done(&spec[k], move[k], m.s[k].global_blocks, &fin);
+            ZCHECK(zk_spec_finalize_set(&fin, zpic_stream()));
             for (int k = 0; k < n; k++) due[k] = sort_due(&spec[k]);
             for (int k = 0; k < n; k++)
                 if (due[k]) sort_now(&spec[k], 0);

@@ -54,6 +54,13 @@ __global__ void finalize_kernel(zk_spec_ctr *ctr, const double *epart, int npart
     finalize_body(ctr, epart, nparts, qm, dx0, dx1);
 }
 
+/* one block per species */
+__global__ void finalize_set_kernel(zk_finalize_set f)
+{
+    const int k = blockIdx.x;
+    finalize_body(f.ctr[k], f.epart[k], f.nparts[k], f.qm[k], f.dx0[k], f.dx1[k]);
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -116,6 +123,13 @@ extern "C" int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int npart
                                        double dx0, double dx1, zk_stream_t s)
 {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, ctr, epart, nparts, qm, dx0, dx1);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_spec_finalize_set(const zk_finalize_set *f, zk_stream_t s)
+{
+    if (f->n < 1 || f->n > ZK_FUSE_MAX) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(finalize_set_kernel, dim3(f->n), dim3(256), 0, s, *f);
     ZK_LAUNCH_CHECK();
 }
 

@@ -95,6 +95,15 @@ int zk_push(const zk_push_args *a, zk_stream_t s);
 /* fused tile pass over m->n species (fixed-point windows), then the
  * global pass of each species that has an unsorted tail */
 int zk_push_species(const zk_push_multi *m, zk_stream_t s);
+/* energy and push count of up to ZK_FUSE_MAX species in one launch */
+typedef struct zk_finalize_set {
+    int n;
+    zk_spec_ctr *ctr[ZK_FUSE_MAX];
+    const double *epart[ZK_FUSE_MAX];
+    int nparts[ZK_FUSE_MAX];
+    double qm[ZK_FUSE_MAX], dx0[ZK_FUSE_MAX], dx1[ZK_FUSE_MAX];
+} zk_finalize_set;
+int zk_spec_finalize_set(const zk_finalize_set *f, zk_stream_t s);
 int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int nparts,
                             double qm, double dx0, double dx1, zk_stream_t s);
 
