@@ -15,13 +15,16 @@ constexpr int LX = BX + 3, LY = BY + 3;
 
 typedef zk_yee_k YeeK; /* (dt/2)/dx, (dt/2)/dy, dt/dx, dt/dy, dt */
 
-/* One BX x BY tile of output nodes, tile (tx, ty) of the buffer. */
+/* One BX x BY tile of output nodes, tile (tx, ty) of the buffer.  The
+ * half and whole steps update the staged B and E in place: a node's new
+ * value depends on its own old value and on neighbours of the other field,
+ * which the previous step finished before the barrier (three LDS arrays,
+ * 24 KB, instead of five). */
 __device__ inline void yee_tile(const float *__restrict__ E, const float *__restrict__ B,
                                 const float *__restrict__ J, float *__restrict__ Eo, float *__restrict__ Bo, int nx,
                                 int ny, int nrow, const YeeK &k, const int tx, const int ty)
 {
     __shared__ float e[3][LY][LX], b[3][LY][LX], jj[3][LY][LX];
-    __shared__ float b1[3][LY][LX], e1[3][LY][LX];
     const int bw = nx + 3, bh = ny + 3;
     const int bi0 = tx * BX, bj0 = ty * BY;
     const int tid = threadIdx.x;
@@ -42,31 +45,23 @@ __device__ inline void yee_tile(const float *__restrict__ E, const float *__rest
     for (int m = tid; m < (LX - 1) * (LY - 1); m += blockDim.x) {
         const int u = m % (LX - 1), v = m / (LX - 1);
         const int i = bi0 - 2 + u, j = bj0 - 2 + v; /* raw indices */
-        float bx = b[0][v][u], by = b[1][v][u], bz = b[2][v][u];
         if (i >= -1 && i <= nx && j >= -1 && j <= ny) {
-            bx += (-k.dty2 * (e[2][v + 1][u] - e[2][v][u]));
-            by += (k.dtx2 * (e[2][v][u + 1] - e[2][v][u]));
-            bz += (-k.dtx2 * (e[1][v][u + 1] - e[1][v][u]) + k.dty2 * (e[0][v + 1][u] - e[0][v][u]));
+            b[0][v][u] += (-k.dty2 * (e[2][v + 1][u] - e[2][v][u]));
+            b[1][v][u] += (k.dtx2 * (e[2][v][u + 1] - e[2][v][u]));
+            b[2][v][u] += (-k.dtx2 * (e[1][v][u + 1] - e[1][v][u]) + k.dty2 * (e[0][v + 1][u] - e[0][v][u]));
         }
-        b1[0][v][u] = bx;
-        b1[1][v][u] = by;
-        b1[2][v][u] = bz;
     }
     __syncthreads();
     /* E1 = E + dt * (curl B1 - J) on u in [1, LX-1), v in [1, LY-1) */
     for (int m = tid; m < (LX - 2) * (LY - 2); m += blockDim.x) {
         const int u = 1 + m % (LX - 2), v = 1 + m / (LX - 2);
         const int i = bi0 - 2 + u, j = bj0 - 2 + v;
-        float ex = e[0][v][u], ey = e[1][v][u], ez = e[2][v][u];
         if (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1) {
-            ex += (+k.dty * (b1[2][v][u] - b1[2][v - 1][u])) - k.dt * jj[0][v][u];
-            ey += (-k.dtx * (b1[2][v][u] - b1[2][v][u - 1])) - k.dt * jj[1][v][u];
-            ez += (+k.dtx * (b1[1][v][u] - b1[1][v][u - 1]) - k.dty * (b1[0][v][u] - b1[0][v - 1][u])) -
-                  k.dt * jj[2][v][u];
+            e[0][v][u] += (+k.dty * (b[2][v][u] - b[2][v - 1][u])) - k.dt * jj[0][v][u];
+            e[1][v][u] += (-k.dtx * (b[2][v][u] - b[2][v][u - 1])) - k.dt * jj[1][v][u];
+            e[2][v][u] += (+k.dtx * (b[1][v][u] - b[1][v][u - 1]) - k.dty * (b[0][v][u] - b[0][v - 1][u])) -
+                          k.dt * jj[2][v][u];
         }
-        e1[0][v][u] = ex;
-        e1[1][v][u] = ey;
-        e1[2][v][u] = ez;
     }
     __syncthreads();
     /* B2 = B1 + dt/2 * (-curl E1) and outputs on u in [1, BX+1), v in [1, BY+1) */
@@ -75,19 +70,19 @@ __device__ inline void yee_tile(const float *__restrict__ E, const float *__rest
         const int bi = bi0 - 1 + u, bj = bj0 - 1 + v;
         if (bi >= bw || bj >= bh) continue;
         const int i = bi - 1, j = bj - 1;
-        float bx = b1[0][v][u], by = b1[1][v][u], bz = b1[2][v][u];
+        float bx = b[0][v][u], by = b[1][v][u], bz = b[2][v][u];
         if (i >= -1 && i <= nx && j >= -1 && j <= ny) {
-            bx += (-k.dty2 * (e1[2][v + 1][u] - e1[2][v][u]));
-            by += (k.dtx2 * (e1[2][v][u + 1] - e1[2][v][u]));
-            bz += (-k.dtx2 * (e1[1][v][u + 1] - e1[1][v][u]) + k.dty2 * (e1[0][v + 1][u] - e1[0][v][u]));
+            bx += (-k.dty2 * (e[2][v + 1][u] - e[2][v][u]));
+            by += (k.dtx2 * (e[2][v][u + 1] - e[2][v][u]));
+            bz += (-k.dtx2 * (e[1][v][u + 1] - e[1][v][u]) + k.dty2 * (e[0][v + 1][u] - e[0][v][u]));
         }
         const size_t g = 3 * ((size_t)bi + (size_t)bj * nrow);
         Bo[g + 0] = bx;
         Bo[g + 1] = by;
         Bo[g + 2] = bz;
-        Eo[g + 0] = e1[0][v][u];
-        Eo[g + 1] = e1[1][v][u];
-        Eo[g + 2] = e1[2][v][u];
+        Eo[g + 0] = e[0][v][u];
+        Eo[g + 1] = e[1][v][u];
+        Eo[g + 2] = e[2][v][u];
     }
 }
 
