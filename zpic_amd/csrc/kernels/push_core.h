@@ -419,7 +419,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
             auto step = [&](Part &c, int i) {
                 const int lx = c.ix - tx0, ly = c.iy - ty0;
                 const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
-                const bool live = i < i1 && c.ix != ZK_DEAD;
+                const bool live = i >= i0 && i < i1 && c.ix != ZK_DEAD;
                 list_out(a, i0, i, live && !inside, &nlist);
                 if (live && inside) {
                     int di, dj;
@@ -430,8 +430,12 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                     store_part(P, i, c);
                 }
             };
+            /* The loop starts at the multiple of 32 at or below i0, so each
+             * wave's 64 consecutive slots cover exactly two 128-byte lines
+             * of every array (up to 31 slots of the previous tile are
+             * loaded and left alone): 1.98 -> 1.94 ms per 2048^2 launch. */
             const int bs = blockDim.x, last = i1 - 1;
-            int i = i0 + threadIdx.x;
+            int i = (i0 & ~31) + threadIdx.x;
             Part ca, cb, cc;
             if (i < i1) {
                 ca = load_part(P, i);
