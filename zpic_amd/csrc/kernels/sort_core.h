@@ -140,9 +140,10 @@ __device__ inline void scatter_body(const zk_parts &src, const zk_parts &dst, co
  * 1.32 ms per 2048^2 species). */
 
 /* 64-particle chunks a wave loads before processing them (loads in flight;
- * 2048^2 Weibel re-sort: 1.45 ms with 4, 1.35 ms with 8, 1.31 ms with 12
- * at occupancy 3, 1.36 ms with 16) */
-#define ZK_SORT_UNROLL 8
+ * 2048^2 Weibel re-sort, unaligned chunks: 1.45 ms with 4, 1.35 ms with
+ * 8, 1.31 ms with 12 at occupancy 3, 1.36 ms with 16; aligned chunks:
+ * 1.314 ms with 8, 1.278 ms with 12) */
+#define ZK_SORT_UNROLL 12
 
 /* piece [*lo, *hi) of the tile range [a, b) owned by wave w of nw */
 __device__ inline void wave_piece(int a, int b, int w, int nw, int *lo, int *hi)
