@@ -266,32 +266,51 @@ __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__
 __device__ inline void scan_stay_body(const int *stay, int *move, int *tailn, int *new_off, int *cursor,
                                       zk_spec_ctr *ctr, int ntiles)
 {
-    __shared__ int part[1024];
-    const int per = (ntiles + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per, e = min(ntiles, b + per);
-    int s = 0;
-    for (int i = b; i < e; i++) s += stay[i] + move[i] + tailn[i];
-    part[threadIdx.x] = s;
+    /* chunks of blockDim.x consecutive tiles, read and written coalesced:
+     * a wave scan by shuffles, a scan of the wave totals, and a running
+     * carry from chunk to chunk */
+    __shared__ int wsum[1024 / ZK_WAVE];
+    __shared__ int carry_s;
+    const int lane = __lane_id(), w = threadIdx.x / ZK_WAVE, nw = blockDim.x / ZK_WAVE;
+    if (threadIdx.x == 0) carry_s = 0;
     __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        int v = (threadIdx.x >= (unsigned)o) ? part[threadIdx.x - o] : 0;
+    for (int base = 0; base < ntiles; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        const int st = i < ntiles ? stay[i] : 0, mv = i < ntiles ? move[i] : 0, tl = i < ntiles ? tailn[i] : 0;
+        const int c = st + mv + tl;
+        int x = c; /* inclusive scan within the wave */
+        for (int o = 1; o < ZK_WAVE; o <<= 1) {
+            const int y = __shfl_up(x, o, ZK_WAVE);
+            if (lane >= o) x += y;
+        }
+        if (lane == ZK_WAVE - 1) wsum[w] = x;
         __syncthreads();
-        part[threadIdx.x] += v;
+        if (w == 0) {
+            int v = lane < nw ? wsum[lane] : 0;
+            for (int o = 1; o < ZK_WAVE; o <<= 1) {
+                const int y = __shfl_up(v, o, ZK_WAVE);
+                if (lane >= o) v += y;
+            }
+            if (lane < nw) wsum[lane] = v;
+        }
+        __syncthreads();
+        const int carry = carry_s;
+        const int run = carry + (w > 0 ? wsum[w - 1] : 0) + x - c;
+        if (i < ntiles) {
+            new_off[i] = run;
+            cursor[i] = run + st + mv;
+            move[i] = 0;
+            tailn[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + wsum[nw - 1];
         __syncthreads();
     }
-    int run = part[threadIdx.x] - s;
-    for (int i = b; i < e; i++) {
-        const int c = stay[i] + move[i] + tailn[i];
-        new_off[i] = run;
-        cursor[i] = run + stay[i] + move[i];
-        move[i] = 0;
-        tailn[i] = 0;
-        run += c;
-    }
-    if (threadIdx.x == blockDim.x - 1) {
-        new_off[ntiles] = part[threadIdx.x];
-        ctr->np_total = part[threadIdx.x];
-        ctr->n_sorted = part[threadIdx.x];
+    if (threadIdx.x == 0) {
+        const int total = carry_s;
+        new_off[ntiles] = total;
+        ctr->np_total = total;
+        ctr->n_sorted = total;
         ctr->n_dead = 0;
     }
 }
