@@ -231,7 +231,7 @@ __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__
                 const bool st = key == block;
                 nstay += __popcll(__ballot(st));
                 const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
-                wave_slot_add(slot, lc);
+                if (slot >= 0) atomicAdd(&lc[slot], 1); /* totals only: no ranks needed here */
                 zk_run_reserve((key >= 0 && !st && slot < 0) ? key : -1, tailn);
             }
         }
