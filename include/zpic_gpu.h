@@ -43,6 +43,13 @@ void zpic_sim_set_tiling(t_simulation *sim, int tx, int ty, int halo);
  * identical generator states. */
 void zpic_rand_clear(void);
 
+/* Generator state of the calling thread: 0 (default) draws from the
+ * process-global state, as the reference's single generator
+ * (em2d/random.c:16-17); 1 gives this thread a state of its own, starting
+ * from the default seeds with no kept deviate, so that several threads can
+ * initialise independent simulations at once. */
+void zpic_rand_thread_private(int on);
+
 /* Variant of the float-atomic push kernel (zpic_set_fixed_deposit(0)):
  * 0 (default) keeps every value in registers, 8 requests 8 waves per SIMD
  * with a few register spills, 2 pushes two particles per thread and

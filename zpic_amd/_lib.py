@@ -33,6 +33,7 @@ def load() -> C.CDLL:
         "zpic_set_fixed_deposit": (None, [C.c_int]),
         "zpic_synchronize": (None, []),
         "zpic_rand_clear": (None, []),
+        "zpic_rand_thread_private": (None, [C.c_int]),
         "spec_sync_host": (None, [C.c_void_p]),
         "emf_sync_host": (None, [C.c_void_p]),
         "current_sync_host": (None, [C.c_void_p]),

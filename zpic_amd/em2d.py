@@ -9,9 +9,15 @@ Data access differs in one respect: the reference returned zero-copy
 views of host buffers (em2d.pyx:305-312, 1044-1150, 1567-1619).  Here the
 simulation state lives in GPU memory, so every data property first copies
 the device state into the host mirror and returns a numpy view of that
-mirror.  Writes through `Species.particles` and the EMF field views are
+mirror.  Reading a view costs that copy and nothing more.  Writing through
+a view of `Species.particles` or of an EMF field (item assignment,
+in-place arithmetic, ufuncs with `out=`, `fill`, `np.copyto`, `np.put`,
+`np.place`, `np.putmask`) marks the mirror edited, and edited mirrors are
 uploaded before the next iteration, as in the reference where such writes
-changed the running simulation.
+changed the running simulation.  A view read before an iteration shows the
+state of that time: read the property again after stepping.  Writes that
+bypass numpy's array interface (e.g. through a `np.void` element of the
+particle array, or a raw pointer) must be followed by `mark_edited()`.
 """
 from __future__ import annotations
 
@@ -39,6 +45,73 @@ def _inner(buf_ptr, nx, comp):
     """Interior component view [ny, nx] of a guard-inclusive float3 grid."""
     g = A.grid_view(buf_ptr, nx)
     return g[1:1 + nx[1], 1:1 + nx[0], comp]
+
+
+class _Mirror(np.ndarray):
+    """View of a host mirror that reports writes.
+
+    `_before` (refresh the mirror from the device) runs before a write and
+    `_after` (mark the mirror edited) after it; views taken from a _Mirror
+    (slices, fields) report to the same callbacks.  Reads call neither."""
+
+    _WRITERS = {np.copyto: "dst", np.put: "a", np.place: "arr", np.putmask: "a"}
+
+    def __array_finalize__(self, obj):
+        self._before = getattr(obj, "_before", None)
+        self._after = getattr(obj, "_after", None)
+
+    def _write(self, fn):
+        if self._before is not None:
+            self._before()
+        r = fn()
+        if self._after is not None:
+            self._after()
+        return r
+
+    def __setitem__(self, key, value):
+        self._write(lambda: super(_Mirror, self).__setitem__(key, value))
+
+    def fill(self, value):
+        self._write(lambda: super(_Mirror, self).fill(value))
+
+    def sort(self, *a, **k):
+        self._write(lambda: super(_Mirror, self).sort(*a, **k))
+
+    def __array_ufunc__(self, ufunc, method, *inputs, out=None, **kwargs):
+        args = tuple(x.view(np.ndarray) if isinstance(x, _Mirror) else x for x in inputs)
+        targets = [o for o in (out or ()) if isinstance(o, _Mirror)]
+        if method == "at" and isinstance(inputs[0], _Mirror):
+            targets.append(inputs[0])  # ufunc.at updates its first operand in place
+        if out is not None:
+            kwargs["out"] = tuple(o.view(np.ndarray) if isinstance(o, _Mirror) else o for o in out)
+        for t in targets:
+            if t._before is not None:
+                t._before()
+        r = getattr(ufunc, method)(*args, **kwargs)
+        for t in targets:
+            if t._after is not None:
+                t._after()
+        if out is not None:
+            # the caller's own output arrays (in-place operators rebind to them)
+            return out[0] if len(out) == 1 else tuple(out)
+        return r
+
+    def __array_function__(self, func, types, args, kwargs):
+        name = self._WRITERS.get(func)
+        plain_types = tuple(np.ndarray if issubclass(t, _Mirror) else t for t in types)
+        if name is None:
+            return super().__array_function__(func, plain_types, args, kwargs)
+        dst = args[0] if args else kwargs.get(name)
+        if not isinstance(dst, _Mirror):
+            return super().__array_function__(func, plain_types, args, kwargs)
+        return dst._write(lambda: super(_Mirror, dst).__array_function__(func, plain_types, args, kwargs))
+
+
+def _watched(arr: np.ndarray, before, after) -> np.ndarray:
+    m = arr.view(_Mirror)
+    m._before = before
+    m._after = after
+    return m
 
 
 class Density:
@@ -155,8 +228,24 @@ class Species:
         lib.spec_mark_host_dirty(self._ptr)
 
     def report(self, type: str, *, quants=(), pha_nx=(), pha_range=()):
-        """report(type, ...) - ZDF file output is not provided by this library."""
-        _lib_().spec_report(self._ptr, self._diag_types[type], None, None)
+        """report(type, quants=[], pha_nx=[], pha_range=[]) - write a ZDF
+        diagnostic ("charge", "pha" or "particles") below the current
+        directory (em2d.pyx:268-303)."""
+        rep = self._diag_types[type]
+        if rep == self._diag_types["pha"]:
+            nxa = (C.c_int * 2)(*[int(v) for v in pha_nx])
+            rng = ((C.c_float * 2) * 2)()
+            for a in range(2):
+                for b in range(2):
+                    rng[a][b] = pha_range[a][b]
+            rep = self._pha_quants[quants[0]] + 16 * self._pha_quants[quants[1]] + 0x2000
+            _lib_().spec_report(self._ptr, rep, nxa, C.cast(rng, C.POINTER(C.c_float * 2)))
+        else:
+            _lib_().spec_report(self._ptr, rep, None, None)
+
+    def mark_edited(self):
+        """Upload the host particle buffer before the next iteration."""
+        _lib_().spec_mark_host_dirty(self._ptr)
 
     @property
     def particles(self) -> np.ndarray:
@@ -165,9 +254,7 @@ class Species:
         Writes to the returned array are uploaded before the next iteration.
         """
         self._sync()
-        lib = _lib_()
-        lib.spec_mark_host_dirty(self._ptr)
-        return A.particles_view(self._s)
+        return _watched(A.particles_view(self._s), self._sync, self.mark_edited)
 
     def charge(self) -> np.ndarray:
         """Charge density on the grid, [ny, nx] (guard cells dropped)."""
@@ -403,10 +490,13 @@ class EMF:
     def solver_type(self):
         return "Yee"
 
+    def mark_edited(self):
+        """Upload the host E and B grids before the next iteration."""
+        _lib_().emf_mark_host_dirty(self._ptr)
+
     def _grid(self, which, comp, part=False):
         lib = _lib_()
         lib.emf_sync_host(self._ptr)
-        lib.emf_mark_host_dirty(self._ptr)
         e = self._e
         if part:
             buf = e.ext_fld.E_part_buf if which == "E" else e.ext_fld.B_part_buf
@@ -414,7 +504,10 @@ class EMF:
                 buf = e.E_buf if which == "E" else e.B_buf
         else:
             buf = e.E_buf if which == "E" else e.B_buf
-        return _inner(buf, (e.nx[0], e.nx[1]), comp)
+        view = _inner(buf, (e.nx[0], e.nx[1]), comp)
+        if part and (e.ext_fld.E_part_buf if which == "E" else e.ext_fld.B_part_buf):
+            return view  # fields plus external fields: derived, not state
+        return _watched(view, lambda: lib.emf_sync_host(self._ptr), self.mark_edited)
 
     Ex = property(lambda self: self._grid("E", 0))
     Ey = property(lambda self: self._grid("E", 1))

@@ -15,8 +15,9 @@ drawn from numpy.random.default_rng(seed):
 and its particles from the em2d generator seeded with
 set_rand_seed(12345 + seed, 67890 + seed) (em2d/random.c:16-17 defaults
 offset by the seed).  Members are built directly through the em2d C API,
-on several host threads at once: the generator state is per thread, so
-each member's initial state is the one a single-threaded run would make.
+on several host threads at once: each worker thread draws from a
+generator state of its own (zpic_rand_thread_private), so each member's
+initial state is the one a single-threaded run would make.
 """
 from __future__ import annotations
 
@@ -91,6 +92,7 @@ def build_members(seeds, threads: int = 16, tiles: dict | None = None) -> list[M
     seeds = list(seeds)
 
     def make(s):
+        lib.zpic_rand_thread_private(1)
         m = Member(lib, member_params(s))
         if tiles:
             t = tiles.get(size_class(m.params))
