@@ -40,6 +40,13 @@ def ensure_built() -> None:
         __graft_entry__.build()
 
 
+def host_cores() -> dict:
+    """Cores of the host and of this process's share: the affinity mask,
+    capped at 16, the host cores allotted to one GPU on the GPU boxes."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"host_cpu_count": os.cpu_count(), "affinity": aff, "share": min(16, aff)}
+
+
 def cpu_baseline(sample_nx: int, ppc: int, steps: int, nproc: int) -> dict | None:
     """Reference em2d (its own -Ofast build) on `nproc` host cores at once,
     each running the sample; aggregate particle-steps/s."""
@@ -63,10 +70,14 @@ def cpu_baseline(sample_nx: int, ppc: int, steps: int, nproc: int) -> dict | Non
         except Exception:
             return None
     npart = 2 * sample_nx * sample_nx * ppc * ppc
+    hc = host_cores()
     return {"value": float(sum(rates)), "unit": "particle-steps/s", "cores": nproc, "kind": kind,
             "single_core_value": float(sorted(rates)[len(rates) // 2]),
-            "sample": (f"same physics on a {sample_nx}x{sample_nx} grid ({npart} particles), {steps} steps, "
-                       f"{nproc} independent processes on {nproc} host cores, {wall:.1f} s wall")}
+            "host_cpu_count": hc["host_cpu_count"], "affinity_cores": hc["affinity"],
+            "sample": (f"same physics on a {sample_nx}x{sample_nx} grid ({npart} particles), {steps} steps "
+                       f"(tile sorts every 20 steps: {steps // 20} sorts), {nproc} independent processes on "
+                       f"{nproc} host cores (the GPU's share of the host: {hc['affinity']} cores in the "
+                       f"affinity mask, {hc['host_cpu_count']} on the host), {wall:.1f} s wall")}
 
 
 def copy_probe(nbytes: int) -> int:
@@ -314,6 +325,34 @@ def lwfa_workload(args) -> dict:
     }
 
 
+def secondary_metrics(args) -> dict:
+    """The other headline metrics on their own workloads, summarised: the
+    batched scan (512 members; every 4th member also run sequentially for
+    the sims/s of sequential runs), the adjoint/forward time ratio of the
+    differentiable 256^2 run, and the LWFA particle-steps/s.  Each is the
+    same measurement as its --workload run."""
+    out = {}
+    b_args = argparse.Namespace(**vars(args))
+    b_args.members, b_args.batch_steps, b_args.seq_stride, b_args.tune = 512, 300, 4, False
+    log("bench: secondary metrics: batched scan")
+    b = batch_workload(b_args)
+    out["batch"] = {"sims_per_s": b["value"], "sequential_sims_per_s": b["sequential"]["value"],
+                    "speedup_vs_sequential": b["speedup_vs_sequential"], "members": b["config"]["members"],
+                    "sequential_members_run": b["sequential"]["members_run"],
+                    "particle_steps_per_s": b["particle_steps_per_s"],
+                    "energy_rel_diff_max": b["energy_rel_diff_max"], "workload": b["config"]["workload"]}
+    log("bench: secondary metrics: adjoint")
+    a = adjoint_workload(args)
+    out["adjoint"] = {"adjoint_over_forward": a["value"], "forward_ms_per_step": a["forward_ms_per_step"],
+                      "backward_ms_per_step": a["backward_ms_per_step"],
+                      "recorded_adjoint_over_forward": a["recorded"]["value"], "workload": a["config"]["workload"]}
+    log("bench: secondary metrics: LWFA")
+    w = lwfa_workload(args)
+    out["lwfa"] = {"particle_steps_per_s": w["value"], "ms_per_step": w["ms_per_step"],
+                   "workload": w["config"]["workload"]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,8 +368,11 @@ def main() -> int:
     ap.add_argument("--fixed", type=int, default=-1, help="fixed-point current deposit: 1 on, 0 off (-1: library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-nx", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=10)
-    ap.add_argument("--cpu-procs", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--cpu-steps", type=int, default=40, help="CPU sample steps (40 span two tile sorts)")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="0: every core of the process's affinity mask, at most 16 (one GPU's share)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the batch, adjoint and LWFA measurements of the default line")
     ap.add_argument("--workload", choices=["weibel", "batch", "adjoint", "lwfa"], default="weibel",
                     help="weibel: the 2048^2 Weibel step (default); batch: the batched parameter scan; "
                          "adjoint: backward/forward time of the differentiable 256^2 Weibel run")
@@ -439,9 +481,13 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        nproc = args.cpu_procs or min(16, os.cpu_count() or 1)
+        nproc = args.cpu_procs or host_cores()["share"]
         log(f"bench: CPU baseline on {nproc} host cores")
         cpu = cpu_baseline(args.cpu_nx, ppc, args.cpu_steps, nproc)
+
+    secondary = None
+    if not args.no_secondary:
+        secondary = secondary_metrics(args)
 
     out = {
         "metric": "particle-steps/s",
@@ -474,6 +520,7 @@ def main() -> int:
         "vs_cpu_single_core": (value / cpu["single_core_value"]) if cpu else None,
         "kernels": kt,
         "overflow_pushes": overflow,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
     return 0

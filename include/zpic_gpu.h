@@ -29,9 +29,11 @@ extern "C" {
 void zpic_set_device(int device);
 
 /* Tile shape for species/currents created afterwards: tx x ty cells with a
- * drift margin of `halo` cells; 0 keeps the default (16, 16, 3).  Tiles
- * must divide the grid; a side that no preferred tile width divides uses
- * the largest divisor <= 64 that leaves room for the margin. */
+ * drift margin of `halo` cells; 0 keeps the default (16, 16, 3).  Any grid
+ * size is accepted: a side that the preferred width does not divide uses a
+ * divisor within a quarter of that width, or else tiles of that width with
+ * a narrower last tile.  A tile whose window (tile + 2 halo + 3 nodes per
+ * side) would not fit the LDS of one workgroup is shrunk until it does. */
 void zpic_set_tiling(int tx, int ty, int halo);
 /* The same for one simulation only (its species and current), before its
  * first step; 0 entries keep the process-wide setting. */
@@ -73,6 +75,23 @@ void zpic_set_fixed_deposit(int on);
  * fused window is rounded to FP32 once for all species together, so the
  * current differs from the per-species launches at FP32 rounding. */
 void zpic_set_species_fusion(int on);
+
+/* Open (absorbing) y boundaries for a simulation, before its first step.
+ * No counterpart in em2d, whose y boundary is always periodic; the model
+ * is the open boundary of the reference's 1D code (em1d/emf.c:379-406,
+ * em1d/particles.h:77-80):
+ *  - fields: first-order Mur conditions on the tangential components Ex,
+ *    Ez of the guard rows j = -1 and j = ny, applied after the E update of
+ *    each step: E_g(n+1) = E_in(n) + S (E_in(n+1) - E_g(n)) with E_in the
+ *    adjacent interior row and S = (dt - dy) / (dt + dy); the guard rows
+ *    are no longer periodic copies;
+ *  - particles: a particle whose cell leaves [0, ny) is removed;
+ *  - current: no fold along y (the raw guard rows are kept, and y filter
+ *    passes leave the guard rows unchanged); spec_deposit_charge does not
+ *    fold along y.
+ * The x boundary stays periodic, or absorbing with the moving window.
+ * Batches and differentiable runs refuse such simulations. */
+void zpic_sim_set_open_y(t_simulation *sim);
 
 /* Block until all queued GPU work is done. */
 void zpic_synchronize(void);

@@ -1,0 +1,132 @@
+"""The benchmark configurations at their stated shapes, and run-to-run
+reproducibility.
+
+* Weibel at 128 x 128 with two electron species of 8 x 8 particles per cell
+  (2,097,152 particles, em2d/input/weibel.c:10-42 with the benchmark's
+  density): cell indices identical by particle id for 10 steps, E, B and J
+  within 1e-5 relative L2 of the oracle after 100 steps.
+* Laser wakefield with a y-polarised laser (polarization 0, em2d/emf.c:331)
+  and a moving window (em2d/input/lwfa.c:12-65), with the LWFA grid
+  spacing, time step, laser and filter: the same bars over 100 steps.
+* Reproducibility: every J node is an exact fixed-point sum (tile windows
+  in LDS, out-of-window and unsorted-tail pushes in the overflow grid), so
+  two runs of one deck give bit-identical E, B and J and particles, also
+  when most pushes take the out-of-window path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import field_err, gpu_ids, rel_l2
+from em2d_driver import lwfa, weibel
+from test_gpu_parity import CASES, gpu_run, matched, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+
+def _id_sorted(gpu, run, k):
+    p = run.state()[f"part{k}"]
+    ids = gpu_ids(gpu, run, k)
+    return p[np.argsort(ids)], np.sort(ids)
+
+
+def _parity_100(gpu, oracle, cfg, tag):
+    g, o = gpu_run(gpu, cfg, ids=True), oracle_run(oracle, cfg, ids=True)
+    try:
+        for step in range(1, 11):
+            g.step(1)
+            o.step(1)
+            for k in range(len(cfg["species"])):
+                pg, po = matched(gpu, oracle, g, o, k)
+                assert np.array_equal(pg["ix"], po["ix"]) and np.array_equal(pg["iy"], po["iy"]), (tag, step, k)
+        print(f"{tag}: cells identical for 10 steps", flush=True)
+        g.step(90)
+        o.step(90)
+        sg, so = g.state(), o.state()
+        errs = {f: (rel_l2(sg[f], so[f]) if f != "B" else field_err(sg, so, f)) for f in ("E", "B", "J")}
+        print(f"{tag}: step 100 relative L2 {errs}", flush=True)
+        for f in ("E", "J"):
+            assert errs[f] <= 1e-5, f
+        assert rel_l2(sg["B"], so["B"]) <= 1e-5 or errs["B"] <= 1e-5
+        for k in range(len(cfg["species"])):
+            assert len(sg[f"part{k}"]) == len(so[f"part{k}"])
+        assert sg["emf_n_move"] == so["emf_n_move"]
+        assert np.allclose(sg["kin_energy"], so["kin_energy"], rtol=1e-5)
+    finally:
+        g.close()
+        o.close()
+
+
+def test_weibel_128_two_electron_species_8x8(gpu, oracle):
+    cfg = weibel(128, 128, 8, m_q2=-1.0)
+    assert sum(s["ppc"][0] * s["ppc"][1] for s in cfg["species"]) * 128 * 128 == 2_097_152
+    _parity_100(gpu, oracle, cfg, "weibel128")
+
+
+def test_lwfa_y_polarised_moving_window_100_steps(gpu, oracle):
+    # the LWFA spacing, time step, laser and 4-pass compensated filter on an
+    # 800 x 128 window; plasma from x = 17, laser focused at x = 20
+    cfg = lwfa(800, 128, start=17.0, focus=20.0, polarization=0.0)
+    _parity_100(gpu, oracle, cfg, "lwfa_ypol")
+
+
+def _forced_overflow():
+    cfg = weibel(32, 32, 3, uz=0.9, uth=0.4, n_sort=0, m_q2=-1.0)
+    return cfg
+
+
+@pytest.mark.parametrize("name", ["hot_fast", "lwfa_electrons_ions", "forced_overflow"])
+def test_two_runs_are_bit_identical(gpu, name):
+    forced = name == "forced_overflow"
+    cfg = _forced_overflow() if forced else CASES[name]
+    if forced:
+        # tiles with a one-cell drift margin and no scheduled sorts: most
+        # pushes leave their window and deposit into the overflow grid
+        gpu.zpic_set_tiling(8, 8, 1)
+    try:
+        runs = [gpu_run(gpu, cfg, ids=True) for _ in range(2)]
+        try:
+            ovf0 = [gpu.spec_overflow_count(C.byref(r.species[0])) for r in runs]
+            for r in runs:
+                r.step(40)
+            ovf = [gpu.spec_overflow_count(C.byref(r.species[0])) - o for r, o in zip(runs, ovf0)]
+            print(f"{name}: out-of-window pushes of species 0 {ovf}", flush=True)
+            assert ovf[0] == ovf[1]
+            if forced:
+                assert ovf[0] > 0
+            sa, sb = runs[0].state(), runs[1].state()
+            for f in ("E", "B", "J"):
+                assert sa[f].tobytes() == sb[f].tobytes(), f
+            for k in range(len(cfg["species"])):
+                pa, ia = _id_sorted(gpu, runs[0], k)
+                pb, ib = _id_sorted(gpu, runs[1], k)
+                assert np.array_equal(ia, ib)
+                assert pa.tobytes() == pb.tobytes(), k
+        finally:
+            for r in runs:
+                r.close()
+    finally:
+        if forced:
+            gpu.zpic_set_tiling(0, 0, 0)
+
+
+@pytest.mark.parametrize("nx,ny", [(67, 67), (106, 106), (127, 45)])
+def test_any_grid_size(gpu, oracle, nx, ny):
+    """Grid sides with no tile width near 16 dividing them (primes, 2 x 53)
+    run with a narrower last tile; the parity bars are those of every
+    other deck."""
+    cfg = weibel(nx, ny, 2, uth=0.2, n_sort=8, m_q2=-1.0)
+    _parity_100(gpu, oracle, cfg, f"grid{nx}x{ny}")
+
+
+def test_oversized_tile_request_is_shrunk_to_fit_lds(gpu, oracle):
+    """64 x 64 tiles with a 3-cell margin would need 255 KB of LDS for the
+    push window; the library shrinks them to fit and the run matches."""
+    gpu.zpic_set_tiling(64, 64, 3)
+    try:
+        _parity_100(gpu, oracle, weibel(128, 64, 2, m_q2=-1.0), "tile64")
+    finally:
+        gpu.zpic_set_tiling(0, 0, 0)

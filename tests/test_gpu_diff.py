@@ -105,9 +105,9 @@ def test_weibel_subcase_gradients_match_fp64_reference():
     Over 200 steps of the instability FP32 rounding alone moves the
     gradients: autograd through the same restatement run in float32
     differs from the FP64 one by 7e-5 (uz), 4e-4 (uth) and 5e-3 (seed,
-    relative L2), and the GPU's own result moves by up to ~5e-4 between
-    runs (the order of the LDS float atomics of the deposit is not fixed).
-    The bars are 1e-3 for uz, 2e-3 for uth and 1e-2 for the seed; the
+    relative L2); the GPU measured 1.2e-5, 2.5e-4 and 4.4e-3.  The bars
+    are about three times the FP32 floor or the measurement, whichever is
+    larger: 2e-4 for uz, 1.2e-3 for uth and 1e-2 for the seed; the
     short-run test above checks the reverse step itself to 1e-4."""
     cfg = WeibelConfig(nx=32, ny=32, ppc=4, steps=200, ckpt=20, loss_first=150, loss_last=200)
     seed = 1e-3 * np.random.default_rng(11).standard_normal((8, 8))
@@ -125,8 +125,8 @@ def test_weibel_subcase_gradients_match_fp64_reference():
           f"d/duth {float(t_uth.grad):.6e} vs {g_uth:.6e}; seed rel {rel(t_seed.grad.numpy(), g_seed):.2e}",
           flush=True)
     assert abs(float(L) - L64) <= 1e-3 * abs(L64)
-    assert abs(float(t_uz.grad) - g_uz) <= 1e-3 * abs(g_uz)
-    assert abs(float(t_uth.grad) - g_uth) <= 2e-3 * abs(g_uth)
+    assert abs(float(t_uz.grad) - g_uz) <= 2e-4 * abs(g_uz)
+    assert abs(float(t_uth.grad) - g_uth) <= 1.2e-3 * abs(g_uth)
     assert rel(t_seed.grad.numpy(), g_seed) <= 1e-2
 
 
@@ -144,8 +144,11 @@ def test_adjoint_matches_central_differences_of_gpu_runs():
     arithmetic (autograd vs central differences through the FP64
     restatement, 32 x 32 subcase): uz 5e-4 at h = 1e-3; uth 1.3e-2; the
     seed direction 3.5e-3 at h = 1e-3 and 8e-3 at h = 1e-4 (no convergence
-    as h shrinks).  Bars: 1e-3 for uz, 2e-2 for uth and 1e-2 for the seed
-    direction; the exact checks of the reverse step are the FP64 ones
+    as h shrinks).  On the GPU at 256 x 256: uz 7e-6, uth 5e-4, seed
+    direction 1.5e-3.  Bars, about three to four times those: 3e-5 for uz,
+    2e-3 for uth and 5e-3 for the seed direction (the seed direction cannot
+    be held to 1e-3: central differences of this loss are themselves off by
+    more than that); the exact checks of the reverse step are the FP64 ones
     above."""
     cfg = WeibelConfig()  # 256 x 256, 2 x 4x4 ppc, 200 steps, ckpt 20, loss 150-200, 64 x 64 seed
     seed = 1e-3 * np.random.default_rng(5).standard_normal((64, 64))
@@ -173,9 +176,9 @@ def test_adjoint_matches_central_differences_of_gpu_runs():
     g_s = float(np.linalg.norm(g_seed))
     print(f"uz {g_uz:.6e} fd {fd_uz:.6e}; uth {g_uth:.6e} fd {fd_uth:.6e}; |seed grad| {g_s:.6e} fd {fd_s:.6e}",
           flush=True)
-    assert abs(g_uz - fd_uz) <= 1e-3 * abs(fd_uz)
-    assert abs(g_uth - fd_uth) <= 2e-2 * abs(fd_uth)
-    assert abs(g_s - fd_s) <= 1e-2 * abs(fd_s)
+    assert abs(g_uz - fd_uz) <= 3e-5 * abs(fd_uz)
+    assert abs(g_uth - fd_uth) <= 2e-3 * abs(fd_uth)
+    assert abs(g_s - fd_s) <= 5e-3 * abs(fd_s)
 
 
 def test_recorded_trajectory_gives_the_checkpointed_gradients():

@@ -75,3 +75,42 @@ def test_full_size_step_conserves_charge_across_sorts(gpu):
         assert [int(run.species[k].np) for k in range(2)] == [NX * NX * 16] * 2
     finally:
         run.close()
+
+
+def test_full_size_late_phase_with_out_of_window_pushes(gpu):
+    """The benchmark deck (sort every 20 steps) with the filter off, checked
+    after 300 steps: by then the plasma is hot, particles drift past the
+    tile margin between sorts (out-of-window pushes, early sorts), and the
+    same conservation laws must hold over the following steps."""
+    cfg = weibel(NX, NX, 4, n_sort=20, m_q2=-1.0)
+    dx = dy = cfg["box"][0] / NX
+    dt = cfg["dt"]
+    print("fullsize-late: building 2048x2048, 134M particles", flush=True)
+    run = Em2dRun(gpu, cfg)
+    try:
+        q0 = float(run.charge(0)[:NX, :NX].astype(np.float64).sum())
+        run.step(300)
+        ovf0 = [gpu.spec_overflow_count(C.byref(run.species[k])) for k in range(2)]
+        E, _ = fields(gpu, run)
+        r = rho(run)
+        gd0 = gauss_defect(E, r, dx, dy)
+        for n in range(6):
+            run.step(1)
+            E, J = fields(gpu, run)
+            r1 = rho(run)
+            res, scale = continuity_residual(r, r1, J, dt, dx, dy)
+            print(f"fullsize-late: step {run.n} continuity {res:.3e} of {scale:.3e}", flush=True)
+            assert res <= 5e-5 * scale
+            r = r1
+        ovf = [gpu.spec_overflow_count(C.byref(run.species[k])) - ovf0[k] for k in range(2)]
+        print(f"fullsize-late: out-of-window pushes over 6 steps {ovf}", flush=True)
+        assert min(ovf) > 0
+        gd1 = gauss_defect(E, r, dx, dy)
+        one = np.abs(run.charge(0)).max()
+        assert np.abs(gd1 - gd0).max() <= 1e-5 * one
+        q1 = float(run.charge(0)[:NX, :NX].astype(np.float64).sum())
+        assert abs(q1 - q0) <= 1e-6 * abs(q0)
+        gpu.sim_sync_host(C.byref(run.sim))
+        assert [int(run.species[k].np) for k in range(2)] == [NX * NX * 16] * 2
+    finally:
+        run.close()

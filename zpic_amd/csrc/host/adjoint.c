@@ -167,6 +167,8 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
         zpic_fatal("zpic_diff_new: differentiable runs do not support a moving window");
     if (sim->emf.ext_fld.E_type != EMF_FLD_TYPE_NONE || sim->emf.ext_fld.B_type != EMF_FLD_TYPE_NONE)
         zpic_fatal("zpic_diff_new: differentiable runs do not support external fields");
+    if (sim->emf.dev && sim->emf.dev->open_y)
+        zpic_fatal("zpic_diff_new: differentiable runs do not support open y boundaries");
     if (sim->current.smooth.xtype != NONE || sim->current.smooth.ytype != NONE)
         zpic_fatal("zpic_diff_new: differentiable runs do not support current filtering");
     zpic_diff *D = calloc(1, sizeof(*D));
@@ -195,6 +197,11 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
     zpic_current_ensure(&sim->current, NULL);
     const size_t gb = sizeof(float) * 3 * D->nodes;
     const zk_tiles *t = &sim->current.dev->t;
+    /* the reverse push keeps E, B and the 64-bit adjoint windows of a tile
+     * in LDS: 84 bytes per window node */
+    if ((size_t)84 * (size_t)t->WX * (size_t)t->WY > (size_t)160 * 1024)
+        zpic_fatal("zpic_diff_new: tiles of %d x %d cells with margin %d need more LDS than a workgroup has; "
+                   "choose smaller tiles (zpic_sim_set_tiling)", t->TX, t->TY, t->H);
 
     D->cs = calloc((size_t)(D->nck + 1) * (D->ns > 0 ? D->ns : 1), sizeof(spec_ckpt));
     D->cE = calloc((size_t)D->nck + 1, sizeof(float *));

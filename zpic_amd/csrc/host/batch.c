@@ -67,7 +67,9 @@ static void fill_sim(zpic_batch *b, int i)
         M->J[k] = dc->J[k];
     }
     M->region = dc->region;
+    for (int k = 0; k < sim->n_species; k++) zpic_current_jov_register(cur, sim->species[k].q);
     M->Jov = dc->Jov;
+    M->jov_unscale = zpic_current_jov_unscale(cur);
     M->ecur = de->cur;
     M->jcur = dc->jcur;
     M->npass = zpic_current_passes(cur, M->pass_axis, M->pass_sa, M->pass_sb, ZK_MAX_PASSES);
@@ -103,6 +105,7 @@ static void fill_spec(zpic_batch *b, int k)
     a->region = dc->region;
     a->fresh = (b->spec_k[k] == 0);
     a->Jov = dc->Jov;
+    a->jov_scale = zpic_current_jov_scale(cur);
     a->epart = d->epart;
     a->t = d->t;
     a->moving_window = 0;
@@ -299,6 +302,8 @@ zpic_batch *zpic_batch_new(t_simulation *const *sims, int n)
             zpic_fatal("zpic_batch_new: simulation %d uses a moving window, which batches do not support", i);
         if (sim->emf.ext_fld.E_type != EMF_FLD_TYPE_NONE || sim->emf.ext_fld.B_type != EMF_FLD_TYPE_NONE)
             zpic_fatal("zpic_batch_new: simulation %d has external fields, which batches do not support", i);
+        if (sim->emf.dev && sim->emf.dev->open_y)
+            zpic_fatal("zpic_batch_new: simulation %d has open y boundaries, which batches do not support", i);
         for (int j = 0; j < sim->n_species; j++)
             if (sim->species[j].moving_window)
                 zpic_fatal("zpic_batch_new: simulation %d uses a moving window, which batches do not support", i);

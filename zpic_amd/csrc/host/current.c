@@ -6,6 +6,7 @@
  * the windows into J with the periodic fold of the reference's guard cells
  * and applies the digital filter passes (em2d/current.c:171-183, 427-459).
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -67,10 +68,10 @@ void zpic_current_ensure(t_current *c, const zk_tiles *t)
         const size_t nodes = zpic_grid_nodes(c->nx);
         d->J[0] = zpic_dmalloc(sizeof(float3) * nodes);
         d->J[1] = zpic_dmalloc(sizeof(float3) * nodes);
-        d->Jov = zpic_dmalloc(sizeof(float3) * nodes);
+        d->Jov = zpic_dmalloc(sizeof(long long) * 3 * nodes);
         d->region = zpic_dmalloc(sizeof(float) * 3 * (size_t)tt.WX * (size_t)tt.WY * (size_t)tt.ntiles);
         ZCHECK(zk_fill_f32(d->J[0], 3 * nodes, 0.0f, zpic_stream()));
-        ZCHECK(zk_fill_f32(d->Jov, 3 * nodes, 0.0f, zpic_stream()));
+        ZCHECK(zk_fill_f32((float *)d->Jov, 6 * nodes, 0.0f, zpic_stream()));
         d->jcur = 0;
         d->ready = 1;
     } else if (t && (t->TX != d->t.TX || t->TY != d->t.TY || t->H != d->t.H || t->nx != d->t.nx ||
@@ -84,10 +85,32 @@ void current_zero(t_current *c)
     /* em2d/current.c:98-107: every deposit of the step starts from zero */
     zpic_current_ensure(c, NULL);
     struct zpic_dev_current *d = c->dev;
-    ZCHECK(zk_fill_f32(d->Jov, 3 * zpic_grid_nodes(c->nx), 0.0f, zpic_stream()));
+    /* zero bits are the fixed-point zero */
+    ZCHECK(zk_fill_f32((float *)d->Jov, 6 * zpic_grid_nodes(c->nx), 0.0f, zpic_stream()));
     d->fresh = 1;
     d->host_valid = 0;
 }
+
+/* The overflow grid sums out-of-window deposits in 64-bit fixed point with
+ * one scale for every species: Q_J = 2^jov_k, the largest Q among the
+ * species that deposit there (each contribution is O(|q|) <= Q_J).  A
+ * species of larger charge raises Q_J; what the grid already holds is then
+ * divided by the power-of-two ratio with an arithmetic shift, which is the
+ * same in every run. */
+void zpic_current_jov_register(t_current *c, float q)
+{
+    struct zpic_dev_current *d = c->dev;
+    const int k = zpic_fixed_point_exp(q);
+    if (d->jov_set && k <= d->jov_k) return;
+    if (d->jov_set && d->ready)
+        ZCHECK(zk_rescale_fx(d->Jov, 3 * zpic_grid_nodes(c->nx), k - d->jov_k, zpic_stream()));
+    d->jov_k = k;
+    d->jov_set = 1;
+}
+
+float zpic_current_jov_scale(const t_current *c) { return ldexpf(1.0f, 40 - (c->dev->jov_set ? c->dev->jov_k : 0)); }
+
+float zpic_current_jov_unscale(const t_current *c) { return ldexpf(1.0f, (c->dev->jov_set ? c->dev->jov_k : 0) - 40); }
 
 /* Weights of the compensating pass after n binomial passes
  * (em2d/current.c:297-304). */
@@ -140,10 +163,12 @@ void current_update(t_current *c)
     zk_reduce_args a;
     a.region = d->fresh ? NULL : d->region;
     a.Jov = d->Jov;
+    a.jov_unscale = zpic_current_jov_unscale(c);
     a.J = d->J[d->jcur];
     a.nrow = c->nrow;
     a.t = d->t;
     a.periodic_x = !c->moving_window;
+    a.open_y = d->open_y;
     zpic_ktimer_begin(ZPIC_T_CURRENT);
     ZCHECK(zk_current_reduce(&a, zpic_stream()));
     const int maxp = 2 * (c->smooth.xlevel > 0 ? c->smooth.xlevel : 0) + 2;
@@ -152,7 +177,7 @@ void current_update(t_current *c)
     const int np = zpic_current_passes(c, axis, sa, sb, maxp);
     for (int k = 0; k < np; k++) {
         ZCHECK(zk_smooth(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, axis[k], sa[k], sb[k],
-                         !c->moving_window, zpic_stream()));
+                         !c->moving_window, d->open_y, zpic_stream()));
         d->jcur = 1 - d->jcur;
     }
     zpic_ktimer_end(ZPIC_T_CURRENT);

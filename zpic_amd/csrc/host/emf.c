@@ -429,8 +429,9 @@ void emf_advance(t_emf *emf, const t_current *ccur)
     const int o = 1 - d->cur;
     zpic_ktimer_begin(ZPIC_T_FIELD);
     ZCHECK(zk_yee(d->E[d->cur], d->B[d->cur], zpic_current_J(cur), d->E[o], d->B[o], emf->nx[0], emf->nx[1],
-                  emf->nrow, emf->dt, emf->dx[0], emf->dx[1], zpic_stream()));
-    ZCHECK(zk_field_guards(d->E[o], d->B[o], emf->nx[0], emf->nx[1], emf->nrow, !emf->moving_window, zpic_stream()));
+                  emf->nrow, emf->dt, emf->dx[0], emf->dx[1], d->open_y, zpic_stream()));
+    ZCHECK(zk_field_guards(d->E[o], d->B[o], emf->nx[0], emf->nx[1], emf->nrow, !emf->moving_window, d->open_y,
+                           zpic_stream()));
     zpic_ktimer_end(ZPIC_T_FIELD);
     d->cur = o;
     part_fields(emf);

@@ -675,10 +675,13 @@ static void push_args(t_species *s, t_emf *emf, t_current *cur, zk_push_args *pa
     a.nrow = emf->nrow;
     a.region = dc->region;
     a.fresh = dc->fresh;
+    zpic_current_jov_register(cur, s->q);
     a.Jov = dc->Jov;
+    a.jov_scale = zpic_current_jov_scale(cur);
     a.epart = d->epart;
     a.t = d->t;
     a.moving_window = s->moving_window;
+    a.open_y = d->open_y;
     /* constants with the reference's precision (em2d/particles.c:1111-1117) */
     a.tem = 0.5 * s->dt / s->m_q;
     a.dt_dx = s->dt / s->dx[0];
@@ -786,6 +789,9 @@ void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur)
         int move[ZK_FUSE_MAX];
         memset(&m, 0, sizeof(m));
         m.n = n;
+        /* every species' charge is covered by the overflow grid's scale
+         * before any arguments are formed */
+        for (int k = 0; k < n; k++) zpic_current_jov_register(cur, spec[k].q);
         for (int k = 0; k < n; k++) push_args(&spec[k], emf, cur, &m.s[k], &move[k]);
         for (int k = 1; k < n && fuse; k++)
             fuse = same_tiles(&m.s[k].t, &m.s[0].t) && m.s[k].fx_scale == m.s[0].fx_scale;
@@ -860,7 +866,8 @@ void spec_deposit_charge(const t_species *cs, float *charge)
     free(h);
     if (!s->moving_window)
         for (int j = 0; j < s->nx[1] + 1; j++) charge[j * nrow] += charge[s->nx[0] + j * nrow];
-    for (int i = 0; i < s->nx[0] + 1; i++) charge[i] += charge[i + s->nx[1] * nrow];
+    if (!d->open_y)
+        for (int i = 0; i < s->nx[0] + 1; i++) charge[i] += charge[i + s->nx[1] * nrow];
 }
 
 static float pha_quantity(const t_species *s, int i, int q)

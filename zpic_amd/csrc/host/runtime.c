@@ -52,7 +52,7 @@ int zpic_species_fusion(void)
     return g_fuse_species;
 }
 
-void zpic_fixed_point_scale(float q, float *scale, float *unscale)
+int zpic_fixed_point_exp(float q)
 {
     /* Q = 2^k, the power of two at or above |q| */
     int k = 0;
@@ -61,6 +61,12 @@ void zpic_fixed_point_scale(float q, float *scale, float *unscale)
         frexpf(aq, &k); /* aq = f 2^k, f in [0.5, 1) */
         if (ldexpf(1.0f, k - 1) == aq) k -= 1;
     }
+    return k;
+}
+
+void zpic_fixed_point_scale(float q, float *scale, float *unscale)
+{
+    const int k = zpic_fixed_point_exp(q);
     *scale = ldexpf(1.0f, 40 - k);
     *unscale = ldexpf(1.0f, k - 40);
 }
@@ -156,34 +162,42 @@ void zpic_synchronize(void) { zpic_sync(); }
 
 size_t zpic_grid_nodes(const int nx[2]) { return (size_t)(nx[0] + 3) * (size_t)(nx[1] + 3); }
 
-/* Tile width for one axis: the preferred width if it divides n, otherwise
- * the divisor of n closest to it within [halo + 3, 64]. */
-static int pick_tile(int n, int want, int halo)
+/* Tile width for one axis: the preferred width when it divides n, else the
+ * divisor of n closest to it within a quarter of it, else the preferred
+ * width itself, with a narrower last tile (the tile count is rounded up);
+ * never wider than n. */
+static int pick_tile(int n, int want)
 {
-    if (want <= n && n % want == 0) return want;
+    if (want >= n) return n;
+    if (n % want == 0) return want;
     int best = -1;
-    for (int d = halo + 3; d <= 64 && d <= n; d++) {
-        if (n % d) continue;
+    for (int d = want - want / 4; d <= want + want / 4; d++) {
+        if (d < 1 || n % d) continue;
         if (best < 0 || abs(d - want) < abs(best - want)) best = d;
     }
-    if (best < 0 && n <= 64) best = n;
-    return best;
+    return best > 0 ? best : want;
 }
 
 void zpic_tiles_for(int nx, int ny, zk_tiles *t) { zpic_tiles_pref(nx, ny, 0, 0, 0, t); }
 
+/* Bytes of LDS the fixed-point push needs per window node: E and B (24)
+ * and the 64-bit current window (24). */
+#define ZPIC_PUSH_LDS_PER_NODE 48
+#define ZPIC_LDS_MAX (160 * 1024)
+
 void zpic_tiles_pref(int nx, int ny, int want_x, int want_y, int want_h, zk_tiles *t)
 {
-    const int wx = want_x > 0 ? want_x : g_tx, wy = want_y > 0 ? want_y : g_ty;
-    int h = want_h > 0 ? want_h : g_halo;
-    int tx = -1, ty = -1;
-    for (; h >= 0; h--) {
-        tx = pick_tile(nx, wx, h);
-        ty = pick_tile(ny, wy, h);
-        if (tx >= h + 3 && ty >= h + 3) break;
+    if (nx < 1 || ny < 1) zpic_fatal("grid %d x %d has no cells", nx, ny);
+    int wx = want_x > 0 ? want_x : g_tx, wy = want_y > 0 ? want_y : g_ty;
+    const int h = want_h > 0 ? want_h : g_halo;
+    /* the push's window must fit the LDS of one workgroup: shrink the
+     * larger side of the requested tile until it does */
+    while ((size_t)ZPIC_PUSH_LDS_PER_NODE * (size_t)(wx + 2 * h + 3) * (size_t)(wy + 2 * h + 3) > ZPIC_LDS_MAX) {
+        if (wx <= 1 && wy <= 1) zpic_fatal("tile margin %d leaves no room for a tile in the LDS", h);
+        if (wx >= wy) wx = wx > 1 ? wx - 1 : 1;
+        else wy = wy > 1 ? wy - 1 : 1;
     }
-    if (tx < 3 || ty < 3 || h < 0)
-        zpic_fatal("grid %d x %d has no tile width in [3, 64] dividing it", nx, ny);
+    const int tx = pick_tile(nx, wx), ty = pick_tile(ny, wy);
     t->nx = nx;
     t->ny = ny;
     t->TX = tx;
@@ -191,8 +205,8 @@ void zpic_tiles_pref(int nx, int ny, int want_x, int want_y, int want_h, zk_tile
     t->H = h;
     t->WX = tx + 2 * h + 3;
     t->WY = ty + 2 * h + 3;
-    t->ntx = nx / tx;
-    t->nty = ny / ty;
+    t->ntx = (nx + tx - 1) / tx;
+    t->nty = (ny + ty - 1) / ty;
     t->ntiles = t->ntx * t->nty;
 }
 

@@ -83,6 +83,17 @@ void sim_set_moving_window(t_simulation *sim)
     for (int i = 0; i < sim->n_species; i++) sim->species[i].moving_window = 1;
 }
 
+void zpic_sim_set_open_y(t_simulation *sim)
+{
+    /* no counterpart in em2d, whose y boundary is always periodic; the
+     * model is the open boundary of the reference's 1D code (Mur fields,
+     * em1d/emf.c:379-406; particle removal, em1d/particles.h:77-80) */
+    if (sim->emf.iter > 0) zpic_fatal("zpic_sim_set_open_y must be called before the first step");
+    sim->emf.dev->open_y = 1;
+    sim->current.dev->open_y = 1;
+    for (int i = 0; i < sim->n_species; i++) sim->species[i].dev->open_y = 1;
+}
+
 void sim_set_ext_fld(t_simulation *sim, t_emf_ext_fld *ext_fld) { emf_set_ext_fld(&sim->emf, ext_fld); }
 
 void sim_sync_host(t_simulation *sim)

@@ -71,6 +71,7 @@ struct zpic_dev_species {
     struct zpic_dev_species *next_live;
     t_species *owner;
     int pref[3];          /* preferred tile (tx, ty, halo), 0: process-wide setting */
+    int open_y;           /* particles leaving through y are removed */
 };
 
 struct zpic_dev_current {
@@ -80,9 +81,12 @@ struct zpic_dev_current {
     float *J[2];
     int jcur;
     float *region;        /* per-tile current windows */
-    float *Jov;           /* raw current of out-of-window pushes */
+    long long *Jov;       /* raw current of out-of-window pushes (fixed point) */
+    int jov_k;            /* Jov holds multiples of 2^(jov_k - 40) */
+    int jov_set;          /* jov_k was set by a species */
     int fresh;            /* no deposit since current_zero */
     int pref[3];          /* preferred tile (tx, ty, halo), 0: process-wide setting */
+    int open_y;           /* no fold along y (open y boundaries) */
 };
 
 struct zpic_dev_emf {
@@ -96,6 +100,7 @@ struct zpic_dev_emf {
     float3 *Eext_host, *Bext_host; /* the same, evaluated on the host */
     double *epartial, *energy6;
     int nblk;
+    int open_y;           /* Mur absorbing boundaries at the y edges */
 };
 
 /* helpers shared by the host modules */
@@ -108,6 +113,11 @@ void zpic_current_ensure(t_current *cur, const zk_tiles *t);
 const float *zpic_emf_particle_E(t_emf *emf);
 const float *zpic_emf_particle_B(t_emf *emf);
 const float *zpic_current_J(t_current *cur);
+/* the overflow grid's fixed-point scale covers a species of charge q
+ * (raised, with the grid's sums rescaled exactly, when q needs more) */
+void zpic_current_jov_register(t_current *cur, float q);
+float zpic_current_jov_scale(const t_current *cur);   /* 2^(40 - jov_k) */
+float zpic_current_jov_unscale(const t_current *cur); /* 2^(jov_k - 40) */
 
 void zpic_spec_sync_energy(t_species *s);
 void zpic_spec_sort(t_species *s); /* tile sort of the device particles now */

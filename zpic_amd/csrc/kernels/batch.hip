@@ -23,6 +23,11 @@
 
 using namespace zk;
 
+/* waves per SIMD the fixed-point tile push is compiled for */
+#ifndef ZK_PUSH_OCC
+#define ZK_PUSH_OCC 4
+#endif
+
 namespace {
 
 __device__ inline zk_push_args spec_args(const zk_bspec &S, const zk_bsim &M)
@@ -46,7 +51,7 @@ __global__ void fill_jov_kernel(const zk_bsim *sims, const int *items)
     const zk_bsim &M = sims[s];
     const size_t n = 3 * (size_t)(M.nx + 3) * (M.ny + 3);
     const size_t i = (size_t)b * blockDim.x + threadIdx.x;
-    if (i < n) M.Jov[i] = 0.0f;
+    if (i < n) M.Jov[i] = 0;
 }
 
 template <int MIN_WAVES, bool FX>
@@ -78,10 +83,12 @@ __global__ void reduce_batch(const zk_bsim *sims, const int *items)
     zk_reduce_args r;
     r.region = M.deposited ? M.region : nullptr;
     r.Jov = M.Jov;
+    r.jov_unscale = M.jov_unscale;
     r.J = j_after(M, 0);
     r.nrow = M.nrow;
     r.t = M.t;
     r.periodic_x = 1;
+    r.open_y = 0;
     reduce_node(r, items[2 * blockIdx.x + 1] * blockDim.x + threadIdx.x);
 }
 
@@ -178,7 +185,7 @@ extern "C" int zk_batch_push(const zk_bspec *specs, const zk_bsim *sims, const i
 {
     if (ntile > 0) {
         if (fixed)
-            hipLaunchKernelGGL((push_tile_batch<5, true>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
+            hipLaunchKernelGGL((push_tile_batch<ZK_PUSH_OCC, true>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
         else if (waves == 8)
             hipLaunchKernelGGL((push_tile_batch<8, false>), dim3(ntile), dim3(256), lds, s, specs, sims, tile_items);
         else
@@ -192,7 +199,7 @@ extern "C" int zk_batch_prepare_push(size_t lds)
 {
     if (lds <= 65536) return 0;
     const void *fns[3] = {(const void *)push_tile_batch<1, false>, (const void *)push_tile_batch<8, false>,
-                          (const void *)push_tile_batch<5, true>};
+                          (const void *)push_tile_batch<ZK_PUSH_OCC, true>};
     for (int k = 0; k < 3; k++) {
         const int e = (int)hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e) return e;

@@ -23,10 +23,11 @@ __device__ inline void reduce_node(const zk_reduce_args &a, const int node)
     if (node >= bw * bh) return;
     const int i = node % bw - 1, j = node / bw - 1;
     const int ci = a.periodic_x ? zk_wrap(i, t.nx) : i;
-    const int cj = zk_wrap(j, t.ny);
+    const int cj = a.open_y ? j : zk_wrap(j, t.ny);
     const int n = t.WX * t.WY;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (int ky = -1; ky <= 1; ky++) {
+        if (a.open_y && ky != 0) continue;
         const int ry = cj + ky * t.ny;
         if (ry < -1 || ry > t.ny + 1) continue;
         const int ty_lo = max(0, floor_div(ry + 1 + t.H - t.WY, t.TY) + 1);
@@ -47,10 +48,10 @@ __device__ inline void reduce_node(const zk_reduce_args &a, const int node)
                     sz += w[2 * n];
                 }
             }
-            const float *o = a.Jov + 3 * ((rx + 1) + (ry + 1) * a.nrow);
-            sx += o[0];
-            sy += o[1];
-            sz += o[2];
+            const long long *o = a.Jov + 3 * ((rx + 1) + (ry + 1) * a.nrow);
+            sx += (float)o[0] * a.jov_unscale;
+            sy += (float)o[1] * a.jov_unscale;
+            sz += (float)o[2] * a.jov_unscale;
         }
     }
     float *out = a.J + 3 * (size_t)node;
@@ -67,7 +68,8 @@ __device__ inline void stencil(float sa, float sb, const float *l, const float *
 }
 
 __device__ inline void smooth_node(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
-                                   int nrow, int axis, float sa, float sb, int periodic_x, const int node)
+                                   int nrow, int axis, float sa, float sb, int periodic_x, const int node,
+                                   const int open_y = 0)
 {
     const int bw = nx + 3, bh = ny + 3;
     if (node >= bw * bh) return;
@@ -83,9 +85,10 @@ __device__ inline void smooth_node(const float *__restrict__ in, float *__restri
             stencil(sa, sb, AT(c - 1, j), AT(c, j), AT(c + 1, j), o);
             return;
         }
-    } else {
+    } else if (jy_in || !open_y) {
         /* kernel_y: interior nodes; then y guards of every column copy the
-         * final value of their image row */
+         * final value of their image row (with open y boundaries the guard
+         * rows keep their values, below) */
         const int c = jy_in ? j : zk_wrap(j, ny);
         if (ix_in) {
             stencil(sa, sb, AT(i, c - 1), AT(i, c), AT(i, c + 1), o);

@@ -25,14 +25,15 @@ namespace {
 
 __global__ void __launch_bounds__(256) yee_kernel(const float *__restrict__ E, const float *__restrict__ B,
                                                   const float *__restrict__ J, float *__restrict__ Eo,
-                                                  float *__restrict__ Bo, int nx, int ny, int nrow, YeeK k)
+                                                  float *__restrict__ Bo, int nx, int ny, int nrow, YeeK k,
+                                                  int open_y)
 {
-    yee_tile(E, B, J, Eo, Bo, nx, ny, nrow, k, blockIdx.x, blockIdx.y);
+    yee_tile(E, B, J, Eo, Bo, nx, ny, nrow, k, blockIdx.x, blockIdx.y, open_y);
 }
 
-__global__ void guard_kernel(float *E, float *B, int nx, int ny, int nrow, int periodic_x)
+__global__ void guard_kernel(float *E, float *B, int nx, int ny, int nrow, int periodic_x, int open_y)
 {
-    guard_node(E, B, nx, ny, nrow, periodic_x, blockIdx.x * blockDim.x + threadIdx.x);
+    guard_node(E, B, nx, ny, nrow, periodic_x, blockIdx.x * blockDim.x + threadIdx.x, open_y);
 }
 
 __global__ void move_window_kernel(const float *E, const float *B, float *Eo, float *Bo, int nx, int ny, int nrow)
@@ -179,7 +180,7 @@ extern "C" int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, i
 }
 
 extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo, int nx, int ny,
-                             int nrow, float dt, float dx, float dy, zk_stream_t s)
+                             int nrow, float dt, float dx, float dy, int open_y, zk_stream_t s)
 {
     YeeK k;
     const float dt2 = dt / 2.0f;
@@ -188,15 +189,18 @@ extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo,
     k.dtx = dt / dx;
     k.dty = dt / dy;
     k.dt = dt;
+    k.mur_y = (dt - dy) / (dt + dy);
     dim3 grid((nx + 3 + BX - 1) / BX, (ny + 3 + BY - 1) / BY);
-    hipLaunchKernelGGL(yee_kernel, grid, dim3(256), 0, s, E, B, J, Eo, Bo, nx, ny, nrow, k);
+    hipLaunchKernelGGL(yee_kernel, grid, dim3(256), 0, s, E, B, J, Eo, Bo, nx, ny, nrow, k, open_y);
     ZK_LAUNCH_CHECK();
 }
 
-extern "C" int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x, zk_stream_t s)
+extern "C" int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x, int open_y,
+                               zk_stream_t s)
 {
     const int n = 3 * (nx + 3) + 3 * ny;
-    hipLaunchKernelGGL(guard_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, B, nx, ny, nrow, periodic_x);
+    hipLaunchKernelGGL(guard_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, B, nx, ny, nrow, periodic_x,
+                       open_y);
     ZK_LAUNCH_CHECK();
 }
 

@@ -13,21 +13,28 @@
  * workgroup stages the E and B values of its window (the tile plus an H
  * cell drift margin plus the 1-below/2-above stencil) into LDS, pushes the
  * particles of its tile range with gathers from LDS, accumulates the
- * current of its window in LDS with ds_add_f32, and finally stores the
- * window to its own slot of `region` with plain coalesced stores.  No two
+ * current of its window in LDS (64-bit fixed point with ds_add_u64 by
+ * default, float ds_add_f32 on request), and finally stores the window to
+ * its own slot of `region` with plain coalesced stores.  No two
  * workgroups write the same address, so no global atomics are needed; the
  * windows are summed into J afterwards by current.hip in a fixed order.
  *
  * A particle that has drifted more than H cells out of its tile since the
  * last sort is listed by the tile pass and pushed by a second kernel on the
  * global path: gathers from the guard-inclusive field buffers exactly as
- * the reference indexes them, and deposits with atomics into the raw
- * overflow grid `Jov`.  The same kernel pushes the unsorted tail
- * (particles injected by the moving window since the last sort).
+ * the reference indexes them, and deposits into the raw overflow grid
+ * `Jov` with 64-bit fixed-point atomics (exact, order-free sums).  The
+ * same kernel pushes the unsorted tail (particles injected by the moving
+ * window since the last sort).
  */
 #include "push_core.h"
 
 using namespace zk;
+
+/* waves per SIMD the fused push is compiled for */
+#ifndef ZK_PUSH_OCC
+#define ZK_PUSH_OCC 4
+#endif
 
 namespace {
 
@@ -38,7 +45,7 @@ __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args 
 }
 
 /* every species of a simulation in one launch (shared staging and window) */
-__global__ void __launch_bounds__(256, 5) push_species_kernel(zk_push_multi m)
+__global__ void __launch_bounds__(256, ZK_PUSH_OCC) push_species_kernel(zk_push_multi m)
 {
     push_tile_body<1, true>(SpeciesSet{m}, zk_xcd_item(blockIdx.x, m.s[0].t.ntiles));
 }
@@ -61,6 +68,12 @@ __global__ void finalize_set_kernel(zk_finalize_set f)
     finalize_body(f.ctr[k], f.epart[k], f.nparts[k], f.qm[k], f.dx0[k], f.dx1[k]);
 }
 
+__global__ void rescale_fx_kernel(long long *p, size_t n, int shift)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] >>= shift;
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -78,7 +91,7 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
     /* variants: 0 lets the compiler keep every value in registers (about
      * 6 waves per SIMD); 8 asks for 8 waves per SIMD at the price of a few
      * register spills; 2 pushes two particles per thread and iteration */
-    const void *fn = a->fixed         ? (const void *)push_tile_kernel<5, 1, true>
+    const void *fn = a->fixed         ? (const void *)push_tile_kernel<ZK_PUSH_OCC, 1, true>
                      : a->waves == 8 ? (const void *)push_tile_kernel<8, 1, false>
                      : a->waves == 2 ? (const void *)push_tile_kernel<1, 2, false>
                                      : (const void *)push_tile_kernel<1, 1, false>;
@@ -89,7 +102,7 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
     }
     if (a->t.ntiles > 0) {
         if (a->fixed)
-            hipLaunchKernelGGL((push_tile_kernel<5, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else if (a->waves == 8)
             hipLaunchKernelGGL((push_tile_kernel<8, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else if (a->waves == 2)
@@ -139,5 +152,14 @@ extern "C" int zk_fill_f32(float *p, size_t n, float v, zk_stream_t s)
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_rescale_fx(long long *p, size_t n, int shift, zk_stream_t s)
+{
+    if (n == 0 || shift <= 0) return 0;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(rescale_fx_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, shift);
     ZK_LAUNCH_CHECK();
 }

@@ -11,6 +11,12 @@
 #include "common.h"
 
 
+/* 1: the fixed-point push defers the deposits of cell-crossing particles
+ * to a per-wave queue (DepQueue); 0: every lane runs deposit() in place */
+#ifndef ZK_DEFER_CROSSING
+#define ZK_DEFER_CROSSING 1
+#endif
+
 namespace zk {
 
 struct Seg {
@@ -59,12 +65,17 @@ struct LdsCurrentFx {
     __device__ void add(int a, int b, int c, float v) const { atomicAdd(&j[c * n + a + b * w], zk_fx(v, scale)); }
 };
 
+/* Raw overflow grid of out-of-window pushes: 64-bit fixed point at the
+ * grid's own scale (2^40 / Q_J), summed with global integer atomics, so
+ * every node holds an exact sum that does not depend on the order of the
+ * atomics and J is the same in every run. */
 struct GlobalCurrent {
-    float *J;
+    long long *J;
     int nrow;
+    float scale;
     __device__ void add(int a, int b, int c, float v) const
     {
-        unsafeAtomicAdd(&J[3 * (a + b * nrow) + c], v);
+        atomicAdd((unsigned long long *)&J[3 * (a + b * nrow) + c], zk_fx(v, scale));
     }
 };
 
@@ -198,6 +209,89 @@ struct Consts {
     float tem, dt_dx, dt_dy, qnx, qny, q;
 };
 
+/* The motion of a particle that stays in its cell is one segment: the
+ * deposit of deposit() with di = dj = 0, without its split arithmetic. */
+template <class D>
+__device__ inline void deposit_in_cell(const D &J, int ix, int iy, float x0, float y0, float dx, float dy,
+                                       float qnx, float qny, float qvz)
+{
+    Seg a;
+    a.x0 = x0;
+    a.y0 = y0;
+    a.dx = dx;
+    a.dy = dy;
+    a.x1 = x0 + dx;
+    a.y1 = y0 + dy;
+    a.qvz = qvz * 0.5f;
+    a.ix = ix;
+    a.iy = iy;
+    deposit_seg(J, a, qnx, qny);
+}
+
+/* Deferred deposits of the particles that cross a cell edge.
+ *
+ * A wave executes each branch of deposit() for all its lanes as soon as
+ * one lane takes it, and in nearly every wave of 64 some particle crosses
+ * a cell edge: without deferral every wave pays for both splits and three
+ * segment deposits.  Instead, the lanes whose particle stays in its cell
+ * deposit its one segment at once, and the crossing lanes hand their
+ * motion to a per-wave queue held one entry per lane in registers (entries
+ * are moved between lanes with ds_permute).  When the queue cannot take
+ * the next batch it is deposited with deposit() by the occupied lanes, so
+ * the split arithmetic runs on (nearly) full waves.  The fixed-point sums
+ * of the window do not depend on the order of the contributions, so J is
+ * unchanged.  Window-local cells fit 8 bits each (windows are at most
+ * 73 nodes per side), di + 1 and dj + 1 2 bits each. */
+struct DepQueue {
+    float x0, y0, dx, dy, qvz;
+    int code;
+    int n; /* entries held (wave-uniform), in lanes 0 .. n - 1 */
+
+    __device__ void init() { n = 0; }
+
+    template <class D>
+    __device__ void flush(const D &J, const Consts &k)
+    {
+        if (n == 0) return;
+        if ((int)__lane_id() < n) {
+            const int ci = code & 0xff, cj = (code >> 8) & 0xff;
+            const int di = ((code >> 16) & 3) - 1, dj = ((code >> 18) & 3) - 1;
+            deposit(J, ci, cj, di, dj, x0, y0, dx, dy, k.qnx, k.qny, qvz);
+        }
+        n = 0;
+    }
+
+    /* called by every lane of the wave; lanes with `cross` append */
+    template <class D>
+    __device__ void push(bool cross, int ci, int cj, int di, int dj, float px, float py, float mx, float my,
+                         float pq, const D &J, const Consts &k)
+    {
+        const unsigned long long m = __ballot(cross);
+        const int add = __popcll(m);
+        if (add == 0) return;
+        if (n + add > ZK_WAVE) flush(J, k);
+        const int lane = __lane_id();
+        const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        /* lanes that do not append send to a lane outside [n, n + add) */
+        const int dst = ((cross ? n + rank : n + add) & (ZK_WAVE - 1)) * 4;
+        const int c = ci | (cj << 8) | ((di + 1) << 16) | ((dj + 1) << 18);
+        const bool take = lane >= n && lane < n + add;
+        const float rx0 = __int_as_float(__builtin_amdgcn_ds_permute(dst, __float_as_int(px)));
+        const float ry0 = __int_as_float(__builtin_amdgcn_ds_permute(dst, __float_as_int(py)));
+        const float rdx = __int_as_float(__builtin_amdgcn_ds_permute(dst, __float_as_int(mx)));
+        const float rdy = __int_as_float(__builtin_amdgcn_ds_permute(dst, __float_as_int(my)));
+        const float rq = __int_as_float(__builtin_amdgcn_ds_permute(dst, __float_as_int(pq)));
+        const int rc = __builtin_amdgcn_ds_permute(dst, c);
+        x0 = take ? rx0 : x0;
+        y0 = take ? ry0 : y0;
+        dx = take ? rdx : dx;
+        dy = take ? rdy : dy;
+        qvz = take ? rq : qvz;
+        code = take ? rc : code;
+        n += add;
+    }
+};
+
 struct Part {
     int ix, iy;
     float x, y, ux, uy, uz;
@@ -295,8 +389,11 @@ __device__ inline double push_one(const F &S, const D &J, int ci, int cj, float 
 }
 
 /* Boundary conditions after the push (em2d/particles.c:1236-1259); the
- * window shift of spec_move_window (:619-640) is applied first. */
-__device__ inline bool boundaries(int &ix, int &iy, int nx, int ny, int mw, int shift)
+ * window shift of spec_move_window (:619-640) is applied first.  With open
+ * y boundaries a particle that left through y is removed, as the moving
+ * window removes those leaving through x (the open particle boundary of
+ * the reference's 1D code, em1d/particles.h:77-80). */
+__device__ inline bool boundaries(int &ix, int &iy, int nx, int ny, int mw, int shift, int open_y)
 {
     if (mw) {
         ix -= shift;
@@ -307,7 +404,14 @@ __device__ inline bool boundaries(int &ix, int &iy, int nx, int ny, int mw, int 
     } else {
         ix += ((ix < 0) ? nx : 0) - ((ix >= nx) ? nx : 0);
     }
-    iy += ((iy < 0) ? ny : 0) - ((iy >= ny) ? ny : 0);
+    if (open_y) {
+        if (iy < 0 || iy >= ny) {
+            ix = ZK_DEAD;
+            return true;
+        }
+    } else {
+        iy += ((iy < 0) ? ny : 0) - ((iy >= ny) ? ny : 0);
+    }
     return false;
 }
 
@@ -419,17 +523,40 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
              * the range re-read its last particle and are not pushed.  (2048^2
              * Weibel push: 1.16 ms per launch with no load ahead, 1.08 ms with
              * one particle ahead, 1.03-1.05 ms with two; three ahead spills.) */
+            DepQueue dq;
+            dq.init();
             auto step = [&](Part &c, int i) {
                 const int lx = c.ix - tx0, ly = c.iy - ty0;
                 const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
                 const bool live = i >= i0 && i < i1 && c.ix != ZK_DEAD;
                 list_out(a, i0, i, live && !inside, &nlist);
-                if (live && inside) {
+                if constexpr (FX && ZK_DEFER_CROSSING) {
+                    /* deposits of cell-crossing particles are deferred (DepQueue) */
+                    const bool go = live && inside;
+                    const int ci = c.ix - ox, cj = c.iy - oy;
+                    float mx = 0.0f, my = 0.0f, qvz = 0.0f;
+                    int di = 0, dj = 0;
+                    if (go) kin += move_one(lf, ci, cj, c.x, c.y, c.ux, c.uy, c.uz, mx, my, qvz, di, dj, k);
+                    const bool cross = go && (di != 0 || dj != 0);
+                    if (go && !cross) deposit_in_cell(lj, ci, cj, c.x, c.y, mx, my, k.qnx, k.qny, qvz);
+                    dq.push(cross, ci, cj, di, dj, c.x, c.y, mx, my, qvz, lj, k);
+                    if (go) {
+                        float x1 = c.x + mx, y1 = c.y + my;
+                        x1 -= di;
+                        y1 -= dj;
+                        c.x = x1;
+                        c.y = y1;
+                        c.ix += di;
+                        c.iy += dj;
+                        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
+                        store_part(P, i, c);
+                    }
+                } else if (live && inside) {
                     int di, dj;
                     kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
                     c.ix += di;
                     c.iy += dj;
-                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
                     store_part(P, i, c);
                 }
             };
@@ -452,6 +579,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                 cb = load_part(P, min(i + 4 * bs, last));
                 step(cc, i + 2 * bs);
             }
+            if constexpr (FX && ZK_DEFER_CROSSING) dq.flush(lj, k);
         } else {
             for (int i = i0 + threadIdx.x; i < i1; i += 2 * blockDim.x) {
                 const int j = i + blockDim.x;
@@ -482,7 +610,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                     kin += ka;
                     ca.ix += dia;
                     ca.iy += dja;
-                    if (boundaries(ca.ix, ca.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    if (boundaries(ca.ix, ca.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
                     store_part(P, i, ca);
                 }
                 if (pb) {
@@ -495,7 +623,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                     kin += kb;
                     cb.ix += dib;
                     cb.iy += djb;
-                    if (boundaries(cb.ix, cb.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    if (boundaries(cb.ix, cb.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
                     store_part(P, j, cb);
                 }
             }
@@ -509,7 +637,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
             if (threadIdx.x == 0) nls[sp] = nl;
         } else if (nl > 0) {
             const GlobalFields gf = {a.E, a.B, a.nrow};
-            const GlobalCurrent gj = {a.Jov, a.nrow};
+            const GlobalCurrent gj = {a.Jov, a.nrow, a.jov_scale};
             for (int q = threadIdx.x; q < nl; q += blockDim.x) {
                 const int i = a.ovf_list[i0 + q];
                 Part c = load_part(P, i);
@@ -517,7 +645,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                 kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
                 c.ix += di;
                 c.iy += dj;
-                if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
                 store_part(P, i, c);
             }
             if (threadIdx.x == 0) atomicAdd(&a.ctr->n_overflow, nl);
@@ -550,7 +678,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                 const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
                 const zk_parts P = a.p;
                 const GlobalFields gf = {a.E, a.B, a.nrow};
-                const GlobalCurrent gj = {a.Jov, a.nrow};
+                const GlobalCurrent gj = {a.Jov, a.nrow, a.jov_scale};
                 const int i0 = a.tile_off[tile];
                 for (int q = __lane_id(); q < nl; q += ZK_WAVE) {
                     const int i = a.ovf_list[i0 + q];
@@ -559,7 +687,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                     kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
                     c.ix += di;
                     c.iy += dj;
-                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
                     store_part(P, i, c);
                 }
                 for (int o = ZK_WAVE / 2; o > 0; o >>= 1) {
@@ -592,7 +720,7 @@ __device__ inline void push_global_body(const zk_push_args &a, const int block, 
     const zk_tiles &t = a.t;
     const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
     const GlobalFields gf = {a.E, a.B, a.nrow};
-    const GlobalCurrent gj = {a.Jov, a.nrow};
+    const GlobalCurrent gj = {a.Jov, a.nrow, a.jov_scale};
     zk_parts P = a.p;
     double kin = 0.0;
     int ndead = 0, nov = 0;
@@ -609,7 +737,7 @@ __device__ inline void push_global_body(const zk_push_args &a, const int block, 
         kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
         c.ix += di;
         c.iy += dj;
-        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift)) ndead++;
+        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
         store_part(P, i, c);
     }
     const double s = zk_block_sum(kin, red);

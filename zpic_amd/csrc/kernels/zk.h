@@ -67,10 +67,12 @@ typedef struct zk_push_args {
     int nrow;
     float *region;              /* [ntiles][3][WY][WX] current windows */
     int fresh;                  /* 1: overwrite region, 0: accumulate    */
-    float *Jov;                 /* raw current of out-of-window pushes  */
+    long long *Jov;             /* raw current of out-of-window pushes, 64-bit
+                                   fixed point at jov_scale (order-free sums) */
     double *epart;              /* energy partials, one per block        */
     zk_tiles t;
     int moving_window;          /* x absorbing + window shift           */
+    int open_y;                 /* y absorbing (particles leaving are removed) */
     int shift;                  /* 1 when the window moves this step    */
     int *ovf_list;              /* slots of out-of-window particles: tile t
                                    lists its own at tile_off[t] onwards */
@@ -79,6 +81,8 @@ typedef struct zk_push_args {
     int fixed;                  /* 1: fixed-point (int64) current window  */
     float fx_scale;             /* fixed point: 2^40 / Q, Q = 2^ceil(log2|q|) */
     float fx_unscale;           /* fixed point: Q / 2^40                  */
+    float jov_scale;            /* fixed point of Jov: 2^40 / Q_J, Q_J the
+                                   largest Q of the species depositing there */
     float tem, dt_dx, dt_dy, qnx, qny, q;
 } zk_push_args;
 
@@ -91,6 +95,8 @@ typedef struct zk_push_multi {
 } zk_push_multi;
 
 int zk_fill_f32(float *p, size_t n, float v, zk_stream_t s);
+/* divide the n fixed-point values of p by 2^shift (arithmetic shift) */
+int zk_rescale_fx(long long *p, size_t n, int shift, zk_stream_t s);
 int zk_push(const zk_push_args *a, zk_stream_t s);
 /* fused tile pass over m->n species (fixed-point windows), then the
  * global pass of each species that has an unsorted tail */
@@ -125,21 +131,24 @@ int zk_pack(const zk_parts *src, int n, void *aos, zk_stream_t s);
 /* current: window reduction + raw overflow -> folded J; smoothing passes */
 typedef struct zk_reduce_args {
     const float *region;
-    const float *Jov;
+    const long long *Jov;
+    float jov_unscale;          /* Q_J / 2^40 */
     float *J;
     int nrow;
     zk_tiles t;
     int periodic_x;
+    int open_y;                 /* no fold along y: the raw guard rows are kept */
 } zk_reduce_args;
 int zk_current_reduce(const zk_reduce_args *a, zk_stream_t s);
 int zk_smooth(const float *in, float *out, int nx, int ny, int nrow, int axis,
-                     float sa, float sb, int periodic_x, zk_stream_t s);
+                     float sa, float sb, int periodic_x, int open_y, zk_stream_t s);
 
 /* fields */
+/* open_y: first-order Mur absorbing boundaries at the y edges */
 int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo,
-                  int nx, int ny, int nrow, float dt, float dx, float dy, zk_stream_t s);
+                  int nx, int ny, int nrow, float dt, float dx, float dy, int open_y, zk_stream_t s);
 int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x,
-                           zk_stream_t s);
+                           int open_y, zk_stream_t s);
 int zk_move_window(const float *E, const float *B, float *Eo, float *Bo, int nx,
                           int ny, int nrow, zk_stream_t s);
 /* out = F + ext[node] when ext is given, else F + (ux,uy,uz), over every node */
@@ -158,16 +167,19 @@ int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, int qa, int 
 /* ---- batches of independent simulations (batch.hip) ---- */
 #define ZK_MAX_PASSES 16
 
-/* Yee constants: (dt/2)/dx, (dt/2)/dy, dt/dx, dt/dy, dt */
+/* Yee constants: (dt/2)/dx, (dt/2)/dy, dt/dx, dt/dy, dt, and the Mur
+ * coefficient (dt - dy)/(dt + dy) of open y boundaries */
 typedef struct zk_yee_k {
-    float dtx2, dty2, dtx, dty, dt;
+    float dtx2, dty2, dtx, dty, dt, mur_y;
 } zk_yee_k;
 
 /* One simulation of a batch.  ecur/jcur select the current E,B and J
  * buffers and are advanced on the device at the end of every step. */
 typedef struct zk_bsim {
     float *E[2], *B[2], *J[2];
-    float *region, *Jov;
+    float *region;
+    long long *Jov;
+    float jov_unscale;
     int ecur, jcur, npass, deposited;
     int nx, ny, nrow;
     zk_tiles t;

@@ -545,10 +545,14 @@ class Current:
 
 class Simulation:
     """Simulation(nx, box, dt, species=None, report=None, mov_window=False, smooth=None,
-    init_fld=None, ext_fld=None) - em2d simulation (em2d.pyx:1726-2074)."""
+    init_fld=None, ext_fld=None, open_y=False) - em2d simulation (em2d.pyx:1726-2074).
+
+    open_y=True (no counterpart in the reference) makes the y boundaries
+    open: Mur absorbing fields and removal of particles leaving through y
+    (zpic_sim_set_open_y)."""
 
     def __init__(self, nx, box, dt, *, species=None, report=None, mov_window=False, smooth: Smooth | None = None,
-                 init_fld: InitialField | None = None, ext_fld: ExternalField | None = None):
+                 init_fld: InitialField | None = None, ext_fld: ExternalField | None = None, open_y: bool = False):
         lib = _lib_()
         self._sim = _ABI.t_simulation()
         # seeds of a new C run (em2d.pyx:1781, em2d/random.c:16-17)
@@ -575,6 +579,8 @@ class Simulation:
         self._keep = [smooth, init_fld, ext_fld]
         if mov_window:
             lib.sim_set_moving_window(C.byref(self._sim))
+        if open_y:
+            lib.zpic_sim_set_open_y(C.byref(self._sim))
         if smooth is not None:
             lib.sim_set_smooth(C.byref(self._sim), C.byref(smooth._s))
         if init_fld is not None:
@@ -595,6 +601,10 @@ class Simulation:
 
     def set_moving_window(self):
         _lib_().sim_set_moving_window(C.byref(self._sim))
+
+    def set_open_y(self):
+        """Open (absorbing) y boundaries, before the first iteration."""
+        _lib_().zpic_sim_set_open_y(C.byref(self._sim))
 
     def set_smooth(self, smooth: Smooth):
         self._keep.append(smooth)
