@@ -279,11 +279,11 @@ def adjoint_workload(args) -> dict:
 
 
 def lwfa_workload(args) -> dict:
-    """Laser wakefield with a moving window: 4000 x 256 cells, dx = 0.05,
-    dy = 0.2, dt = 0.045; electrons (q/m = -1, 4 x 2 ppc, cold) from
-    x = 20 on; Gaussian laser a0 = 2, omega0 = 10, W0 = 4, FWHM 2, start 17,
-    focus 20, axis 25.6, polarization 0; 4-pass compensated filter along
-    x; 2000 steps (10 untimed)."""
+    """Laser wakefield with a moving window and open y boundaries: 4000 x
+    256 cells, dx = 0.05, dy = 0.2, dt = 0.045; electrons (q/m = -1, 4 x 2
+    ppc, cold) from x = 20 on; Gaussian laser a0 = 2, omega0 = 10, W0 = 4,
+    FWHM 2, start 17, focus 20, axis 25.6, polarization 0; 4-pass
+    compensated filter along x; 2000 steps (10 untimed)."""
     import ctypes as C
 
     from zpic_amd import _lib
@@ -292,10 +292,12 @@ def lwfa_workload(args) -> dict:
     lib = _lib.load()
     lib.zpic_set_push_waves(args.waves)
     nx, ny, dx, dy, dt = 4000, 256, 0.05, 0.2, 0.045
-    log("lwfa: building 4000x256 with moving window")
+    log("lwfa: building 4000x256 with moving window and open y boundaries")
+    lib.zpic_set_device_init(0 if args.host_init else 1)
     e = em2d.Species("electrons", -1.0, [4, 2], density=em2d.Density(type="step", start=20.0))
     sim = em2d.Simulation([nx, ny], [nx * dx, ny * dy], dt, species=[e], mov_window=True,
-                          smooth=em2d.Smooth(xtype="compensated", xlevel=4))
+                          smooth=em2d.Smooth(xtype="compensated", xlevel=4), open_y=True)
+    lib.zpic_set_device_init(0)
     sim.add_laser(em2d.Laser(type="gaussian", start=17.0, fwhm=2.0, a0=2.0, omega0=10.0, W0=4.0, focus=20.0,
                              axis=25.6, polarization=0.0))
     npart = int(e._s.np)
@@ -318,8 +320,9 @@ def lwfa_workload(args) -> dict:
         "steps": steps, "warmup": warm, "ms_per_step": 1e3 * el / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: STEP plasma and Gaussian laser from the em2d initialisers",
-        "config": {"workload": "em2d LWFA 4000x256, dx=0.05 dy=0.2 dt=0.045, moving window, 4x2 ppc from x=20, "
-                               "Gaussian laser a0=2 omega0=10 W0=4 fwhm=2, compensated 4-pass x filter",
+        "config": {"workload": "em2d LWFA 4000x256, dx=0.05 dy=0.2 dt=0.045, moving window, open (Mur) y "
+                               "boundaries, 4x2 ppc from x=20, Gaussian laser a0=2 omega0=10 W0=4 fwhm=2, "
+                               "compensated 4-pass x filter",
                    "particles_at_start": npart},
         "pushes": int(pushes),
     }
@@ -371,6 +374,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40, help="CPU sample steps (40 span two tile sorts)")
     ap.add_argument("--cpu-procs", type=int, default=0,
                     help="0: every core of the process's affinity mask, at most 16 (one GPU's share)")
+    ap.add_argument("--host-init", action="store_true",
+                    help="initialise the particles on the host (default: on the device, zpic_set_device_init)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the batch, adjoint and LWFA measurements of the default line")
     ap.add_argument("--workload", choices=["weibel", "batch", "adjoint", "lwfa"], default="weibel",
@@ -418,14 +423,20 @@ def main() -> int:
     nx = args.nx
     ppc = args.ppc
     dx, dt, uz, uth = 0.1, 0.07, 0.6, 0.1
-    log(f"bench: building Weibel {nx}x{nx}, 2 species x {ppc}x{ppc} ppc")
+    lib.zpic_set_device_init(0 if args.host_init else 1)
+    log(f"bench: building Weibel {nx}x{nx}, 2 species x {ppc}x{ppc} ppc "
+        f"({'host' if args.host_init else 'device'} initialisation)")
+    lib.zpic_synchronize()
     t_init = time.time()
     sp = [em2d.Species("electrons", -1.0, [ppc, ppc], ufl=[0.0, 0.0, uz], uth=[uth] * 3, n_sort=args.n_sort),
           em2d.Species("electrons2", -1.0, [ppc, ppc], ufl=[0.0, 0.0, -uz], uth=[uth] * 3, n_sort=args.n_sort)]
     sim = em2d.Simulation([nx, nx], [nx * dx, nx * dx], dt, species=sp,
                           smooth=em2d.Smooth(xtype="binomial", ytype="binomial", xlevel=1, ylevel=1))
     npart = sum(s._s.np for s in sp)
-    log(f"bench: init {time.time() - t_init:.1f} s, {npart} particles; {args.warmup} warm-up steps")
+    lib.zpic_synchronize()
+    init_s = time.time() - t_init
+    lib.zpic_set_device_init(0)
+    log(f"bench: init {init_s:.2f} s, {npart} particles; {args.warmup} warm-up steps")
     for _ in range(args.warmup):
         sim.iter()
     lib.zpic_synchronize()
@@ -520,6 +531,8 @@ def main() -> int:
         "vs_cpu_single_core": (value / cpu["single_core_value"]) if cpu else None,
         "kernels": kt,
         "overflow_pushes": overflow,
+        "init": {"seconds": init_s, "where": "host" if args.host_init else "device",
+                 "note": "spec_new of both species and sim_new, before the first step (not in the timed region)"},
         "secondary": secondary,
     }
     print(json.dumps(out), flush=True)

@@ -93,6 +93,24 @@ void zpic_set_species_fusion(int on);
  * Batches and differentiable runs refuse such simulations. */
 void zpic_sim_set_open_y(t_simulation *sim);
 
+/* Particle initialisation of spec_new on the device, for the uniform, step
+ * and slab densities (custom densities are always initialised on the
+ * host): 1 on, 0 (default; environment variable ZPIC_DEVICE_INIT=1 turns
+ * it on) off.  The device makes the same lattice positions, draws the
+ * thermal momenta from the reference generator's own stream (continued
+ * from its present state, which is then advanced past the deviates used,
+ * em2d/random.c:48-101) and removes the same per-cell means
+ * (em2d/particles.c:85-354).  The polar method's log is correctly rounded
+ * on the device, which the C library's is not everywhere (it differs in
+ * about 0.06 % of the logs), so a momentum can in rare cases differ from
+ * the host initialisation's in its last bit.  The particles stay on the
+ * device: spec->part is filled by spec_sync_host (before the first step
+ * as after it). */
+void zpic_set_device_init(int on);
+/* logs of the device initialisation whose correct rounding the
+ * double-double evaluation could not certify (expected 0) */
+long long zpic_device_init_ambiguous(void);
+
 /* Block until all queued GPU work is done. */
 void zpic_synchronize(void);
 

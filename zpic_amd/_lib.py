@@ -29,6 +29,8 @@ def load() -> C.CDLL:
         "zpic_set_device": (None, [C.c_int]),
         "zpic_set_tiling": (None, [C.c_int, C.c_int, C.c_int]),
         "zpic_sim_set_open_y": (None, [C.c_void_p]),
+        "zpic_set_device_init": (None, [C.c_int]),
+        "zpic_device_init_ambiguous": (C.c_longlong, []),
         "zpic_sim_set_tiling": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "zpic_set_push_waves": (None, [C.c_int]),
         "zpic_set_fixed_deposit": (None, [C.c_int]),

@@ -17,6 +17,7 @@
 #include "random.h"
 #include "timer.h"
 #include "zpic_internal.h"
+#include "zpic_rng.h"
 #include "zpic_zdf.h"
 
 /* all live species, for the process-wide push counter; spec_new may run
@@ -219,6 +220,183 @@ static void inject_momenta(const t_species *s, t_part *p, int n, zpic_inj_scratc
     }
 }
 
+static void grow_sets(struct zpic_dev_species *d, int need);
+
+/* Logs of the polar method that the device initialisation could not
+ * certify as correctly rounded (expected to stay 0). */
+static long long g_init_ambiguous = 0;
+long long zpic_device_init_ambiguous(void) { return g_init_ambiguous; }
+
+#define ZPIC_MZ 2422800383u /* 36969 * 2^16 - 1 */
+#define ZPIC_MW 1179647999u /* 18000 * 2^16 - 1 */
+
+static uint32_t host_powmod(uint32_t a, uint64_t e, uint32_t m)
+{
+    uint64_t r = 1 % m, b = a % m;
+    while (e) {
+        if (e & 1u) r = (r * b) % m;
+        b = (b * b) % m;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+/* The particles of spec_new made on the device, for the uniform, step and
+ * slab profiles: the same positions, thermal momenta (the generator's
+ * stream continued from its present state, which is advanced past the
+ * deviates used) and mean removal as inject_positions and inject_momenta,
+ * left unsorted in the device set until the species' tiles are chosen
+ * (zpic_species_ensure).  The host mirror is filled on spec_sync_host. */
+static void device_init(t_species *s, const int r[2][2])
+{
+    struct zpic_dev_species *d = s->dev;
+    const int npc = s->ppc[0] * s->ppc[1];
+    const float hx = 1.0f / s->ppc[0], hy = 1.0f / s->ppc[1];
+    const int c0 = r[0][0], ncols = r[0][1] - r[0][0] + 1, rows = r[1][1] - r[1][0] + 1;
+    float *lat = malloc(sizeof(float) * 2 * (size_t)(npc > 0 ? npc : 1));
+    int *tab_i = malloc(sizeof(int) * (size_t)ncols * (size_t)(npc > 0 ? npc : 1));
+    int *tab_k = malloc(sizeof(int) * (size_t)ncols * (size_t)(npc > 0 ? npc : 1));
+    int *cstart = malloc(sizeof(int) * (size_t)ncols), *ccount = malloc(sizeof(int) * (size_t)ncols);
+    if (!lat || !tab_i || !tab_k || !cstart || !ccount) zpic_fatal("spec_new: out of host memory");
+    for (int b = 0, k = 0; b < s->ppc[1]; b++)
+        for (int a = 0; a < s->ppc[0]; a++, k++) {
+            lat[2 * k] = hx * (a + 0.5);
+            lat[2 * k + 1] = hy * (b + 0.5);
+        }
+    /* one row of cells: the (column, lattice point) of each particle, in
+     * the order inject_positions emits them */
+    const float lo = s->density.start / s->dx[0] - s->n_move;
+    const float hi = s->density.end / s->dx[0] - s->n_move;
+    const int cut = s->density.type != UNIFORM, slab = s->density.type == SLAB;
+    int perrow = 0;
+    for (int i = r[0][0]; i <= r[0][1]; i++) {
+        cstart[i - c0] = perrow;
+        for (int k = 0; k < npc; k++)
+            if (!cut || (i + lat[2 * k] > lo && (!slab || i + lat[2 * k] < hi))) {
+                tab_i[perrow] = i;
+                tab_k[perrow] = k;
+                perrow++;
+            }
+        ccount[i - c0] = perrow - cstart[i - c0];
+    }
+    const long long np = (long long)perrow * rows;
+    if (np > ZK_MAX_PARTS) zpic_fatal("species '%s' needs %lld particles; at most %d are supported", s->name, np,
+                                      ZK_MAX_PARTS);
+    s->np = (int)np;
+    spec_grow_buffer(s, s->np);
+    d->cap = 0;
+    d->cur = 0;
+    grow_sets(d, s->np > 0 ? s->np : 1);
+
+    if (np > 0) {
+        zk_init_args a;
+        memset(&a, 0, sizeof(a));
+        a.np = np;
+        a.perrow = perrow;
+        a.row0 = r[1][0];
+        a.rows = rows;
+        a.col0 = c0;
+        a.ncols = ncols;
+        int *dtab = zpic_dmalloc(sizeof(int) * 2 * (size_t)perrow + sizeof(int) * 2 * (size_t)ncols);
+        float *dlat = zpic_dmalloc(sizeof(float) * 2 * (size_t)npc);
+        zpic_h2d(dtab, tab_i, sizeof(int) * (size_t)perrow);
+        zpic_h2d(dtab + perrow, tab_k, sizeof(int) * (size_t)perrow);
+        zpic_h2d(dtab + 2 * perrow, cstart, sizeof(int) * (size_t)ncols);
+        zpic_h2d(dtab + 2 * perrow + ncols, ccount, sizeof(int) * (size_t)ncols);
+        zpic_h2d(dlat, lat, sizeof(float) * 2 * (size_t)npc);
+        a.tab_i = dtab;
+        a.tab_k = dtab + perrow;
+        a.col_start = dtab + 2 * perrow;
+        a.col_count = dtab + 2 * perrow + ncols;
+        a.lat = dlat;
+        const zk_parts *P = &d->set[d->cur];
+        a.ix = P->ix;
+        a.iy = P->iy;
+        a.x = P->x;
+        a.y = P->y;
+        a.ux = P->ux;
+        a.uy = P->uy;
+        a.uz = P->uz;
+        a.stride = s->nx[1];
+        for (int c = 0; c < 3; c++) {
+            a.uth[c] = s->uth[c];
+            a.ufl[c] = s->ufl[c];
+        }
+        ZCHECK(zk_init_positions(&a, zpic_stream()));
+
+        /* thermal momenta: 3 np deviates, the first one kept from before */
+        zpic_rng_state *st = zpic_rng();
+        const long long D = 3 * np;
+        a.z0 = st->z;
+        a.w0 = st->w;
+        a.z1 = 36969u * (st->z & 0xffffu) + (st->z >> 16);
+        a.w1 = 18000u * (st->w & 0xffffu) + (st->w >> 16);
+        a.kept0 = st->have_kept ? 1 : 0;
+        a.kept_in = st->kept;
+        a.pairs = (D - a.kept0 + 1) / 2;
+        const int per = 64;
+        long long nat = (long long)(a.pairs * 1.2732395447351628 * 1.002) + 65536; /* 4/pi attempts per pair */
+        for (;;) {
+            const long long nthr = (nat + per - 1) / per;
+            const int nblk = (int)((nthr + 255) / 256);
+            int *thr = zpic_dmalloc(sizeof(int) * 256 * (size_t)nblk);
+            int *bcnt = zpic_dmalloc(sizeof(int) * (size_t)nblk);
+            ZCHECK(zk_init_rng_count(&a, nat, per, thr, bcnt, nblk, zpic_stream()));
+            int *hb = malloc(sizeof(int) * (size_t)nblk);
+            long long *hoff = malloc(sizeof(long long) * (size_t)nblk);
+            zpic_d2h(hb, bcnt, sizeof(int) * (size_t)nblk);
+            long long tot = 0;
+            for (int b = 0; b < nblk; b++) {
+                hoff[b] = tot;
+                tot += hb[b];
+            }
+            if (tot >= a.pairs) {
+                long long *doff = zpic_dmalloc(sizeof(long long) * (size_t)nblk);
+                zk_init_out *dout = zpic_dmalloc(sizeof(zk_init_out));
+                const zk_init_out zero = {-1, 0.0, 0, 0};
+                zpic_h2d(dout, &zero, sizeof(zero));
+                zpic_h2d(doff, hoff, sizeof(long long) * (size_t)nblk);
+                ZCHECK(zk_init_rng_emit(&a, nat, per, thr, doff, nblk, dout, zpic_stream()));
+                zk_init_out out;
+                zpic_d2h(&out, dout, sizeof(out));
+                zpic_dfree(doff);
+                zpic_dfree(dout);
+                if (out.last_attempt < 0) zpic_fatal("spec_new: device initialisation lost its last deviate pair");
+                /* the generator after the calls of the last attempt used */
+                const uint64_t n = 2 * (uint64_t)out.last_attempt + 2;
+                st->z = (uint32_t)(((uint64_t)host_powmod(36969u, n - 1, ZPIC_MZ) * a.z1) % ZPIC_MZ);
+                st->w = (uint32_t)(((uint64_t)host_powmod(18000u, n - 1, ZPIC_MW) * a.w1) % ZPIC_MW);
+                st->have_kept = out.have_kept;
+                st->kept = out.have_kept ? out.kept : 0.0;
+                g_init_ambiguous += out.ambiguous;
+            }
+            free(hb);
+            free(hoff);
+            zpic_dfree(thr);
+            zpic_dfree(bcnt);
+            if (tot >= a.pairs) break;
+            nat = nat + nat / 16 + 65536;
+        }
+        /* mean removal: keys ix + nx[1] iy */
+        const long long nkeys = (long long)(r[0][1] + 1) + (long long)s->nx[1] * r[1][1];
+        float *mean = zpic_dmalloc(sizeof(float) * 3 * (size_t)nkeys);
+        ZCHECK(zk_init_mean(&a, mean, (int)nkeys, zpic_stream()));
+        zpic_sync();
+        zpic_dfree(mean);
+        zpic_dfree(dtab);
+        zpic_dfree(dlat);
+    }
+    free(lat);
+    free(tab_i);
+    free(tab_k);
+    free(cstart);
+    free(ccount);
+    d->np_upper = s->np;
+    d->dev_unsorted = 1;
+    d->host_dirty = 0;
+    d->host_valid = 0;
+}
+
 void spec_grow_buffer(t_species *s, const int size)
 {
     if (size > s->np_max) {
@@ -276,21 +454,26 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
     s->np = 0;
 
     const int r[2][2] = {{0, nx[0] - 1}, {0, nx[1] - 1}};
-    spec_grow_buffer(s, inject_bound(s, r));
-    s->np = inject_positions(s, r, s->part);
-    inject_momenta(s, s->part, s->np, NULL);
+    struct zpic_dev_species *d = calloc(1, sizeof(*d));
+    s->dev = d;
+    const int dens = s->density.type;
+    if (zpic_device_init() && (dens == UNIFORM || dens == STEP || dens == SLAB)) {
+        device_init(s, r);
+    } else {
+        spec_grow_buffer(s, inject_bound(s, r));
+        s->np = inject_positions(s, r, s->part);
+        inject_momenta(s, s->part, s->np, NULL);
+        d->host_dirty = 1;
+        d->host_valid = 1;
+    }
     s->n_sort = 16;
 
-    struct zpic_dev_species *d = calloc(1, sizeof(*d));
-    d->host_dirty = 1;
-    d->host_valid = 1;
     d->auto_sort = 1;
     d->owner = s;
     pthread_mutex_lock(&g_live_lock);
     d->next_live = g_live;
     g_live = d;
     pthread_mutex_unlock(&g_live_lock);
-    s->dev = d;
 }
 
 static void species_free_device(struct zpic_dev_species *d)
@@ -519,11 +702,14 @@ void zpic_species_ensure(t_species *s)
             long long full = (long long)s->nx[0] * s->nx[1] * s->ppc[0] * s->ppc[1];
             if (full > want) want = (int)full;
         }
-        d->cap = 0;
-        d->cur = 0;
+        if (!d->dev_unsorted) {
+            d->cap = 0;
+            d->cur = 0;
+        }
         grow_sets(d, want);
         d->ctr = zpic_dmalloc(sizeof(zk_spec_ctr));
-        const zk_spec_ctr zero = {0};
+        zk_spec_ctr zero = {0};
+        if (d->dev_unsorted) zero.np_total = d->np_upper; /* made by spec_new, not yet sorted */
         zpic_h2d(d->ctr, &zero, sizeof(zero));
         d->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
         d->tile_off2 = zpic_dmalloc(sizeof(int) * (size_t)(d->t.ntiles + 2));
@@ -537,7 +723,14 @@ void zpic_species_ensure(t_species *s)
         d->epart_cap = d->t.ntiles + ZPIC_GLOBAL_BLOCKS;
         d->epart = zpic_dmalloc(sizeof(double) * (size_t)d->epart_cap);
         d->ready = 1;
-        d->host_dirty = 1;
+        if (d->dev_unsorted) {
+            d->tail_upper = d->np_upper;
+            sort_now(s, 1);
+            d->dev_unsorted = 0;
+            d->host_dirty = 0;
+        } else {
+            d->host_dirty = 1;
+        }
     }
     if (d->host_dirty) upload(s);
 }
@@ -553,6 +746,7 @@ void spec_mark_host_dirty(t_species *s)
 void spec_sync_host(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
+    if (d && d->dev_unsorted) zpic_species_ensure(s);
     if (!d || !d->ready || d->host_valid || d->host_dirty) return;
     zk_spec_ctr c;
     zpic_d2h(&c, d->ctr, sizeof(c));
@@ -847,6 +1041,7 @@ void spec_move_window(t_species *s)
 /* ------------------------------------------------------------------------
  * diagnostics
  * ---------------------------------------------------------------------- */
+
 void spec_deposit_charge(const t_species *cs, float *charge)
 {
     /* CIC deposit on the device, boundary fold on the host

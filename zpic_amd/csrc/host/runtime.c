@@ -28,6 +28,18 @@ static int g_push_waves = 0;
 void zpic_set_push_waves(int waves) { g_push_waves = (waves == 8 || waves == 2) ? waves : 0; }
 int zpic_push_waves(void) { return g_push_waves; }
 
+/* -1: not chosen yet (environment variable ZPIC_DEVICE_INIT, default off) */
+static int g_device_init = -1;
+void zpic_set_device_init(int on) { g_device_init = on ? 1 : 0; }
+int zpic_device_init(void)
+{
+    if (g_device_init < 0) {
+        const char *e = getenv("ZPIC_DEVICE_INIT");
+        g_device_init = (e && e[0] == '1') ? 1 : 0;
+    }
+    return g_device_init;
+}
+
 /* -1: not chosen yet (environment variable ZPIC_FIXED_DEPOSIT, default on) */
 static int g_fixed_deposit = -1;
 void zpic_set_fixed_deposit(int on) { g_fixed_deposit = on ? 1 : 0; }

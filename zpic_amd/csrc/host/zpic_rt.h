@@ -47,6 +47,7 @@ void zpic_tiles_for(int nx, int ny, zk_tiles *t);
 void zpic_tiles_pref(int nx, int ny, int want_x, int want_y, int want_h, zk_tiles *t);
 int zpic_push_waves(void);
 int zpic_fixed_deposit(void);
+int zpic_device_init(void);
 int zpic_species_fusion(void);
 /* fixed-point deposit constants for particle charge q (see push_core.h) */
 void zpic_fixed_point_scale(float q, float *scale, float *unscale);

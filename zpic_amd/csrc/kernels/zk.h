@@ -164,6 +164,41 @@ int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, float q,
 int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, int qa, int qb, float dx0, float dx1, int na,
            int nb, float amin, float bmin, float ra, float rb, float q, float *out, zk_stream_t s);
 
+/* ---- particle initialisation on the device (init.hip) ---- */
+typedef struct zk_init_args {
+    long long np;                       /* particles of the injected range */
+    int perrow;                         /* particles per row of cells */
+    int row0, rows, col0, ncols;        /* injected cell range */
+    const int *tab_i, *tab_k;           /* [perrow] column and lattice point of a row's particles */
+    const int *col_start, *col_count;   /* [ncols] first entry and entries of a column in a row */
+    const float *lat;                   /* [2 ppc] lattice points in the cell */
+    int *ix, *iy;                       /* outputs, particle order */
+    float *x, *y, *ux, *uy, *uz;
+    int stride;                         /* cell key ix + stride iy of the mean removal (nx[1]) */
+    float uth[3], ufl[3];
+    uint32_t z0, w0;                    /* generator state before the species */
+    uint32_t z1, w1;                    /* ... and after its first step */
+    int kept0;                          /* 1: a deviate kept from before is deviate 0 */
+    double kept_in;
+    long long pairs;                    /* pairs of deviates drawn */
+} zk_init_args;
+typedef struct zk_init_out {
+    long long last_attempt;             /* attempt that gave the last pair */
+    double kept;                        /* deviate left over (have_kept) */
+    int have_kept;
+    int ambiguous;                      /* log values too close to a rounding boundary */
+} zk_init_out;
+int zk_init_positions(const zk_init_args *a, zk_stream_t s);
+/* attempts [0, nat) of the polar method, `per` per thread, nblocks blocks
+ * of 256 threads: per-thread prefix in the block, per-block count */
+int zk_init_rng_count(const zk_init_args *a, long long nat, int per, int *thr_off, int *blk_cnt, int nblocks,
+                      zk_stream_t s);
+/* blk_off: exclusive prefix of the block counts */
+int zk_init_rng_emit(const zk_init_args *a, long long nat, int per, const int *thr_off, const long long *blk_off,
+                     int nblocks, zk_init_out *out, zk_stream_t s);
+/* mean (3 nkeys floats) per cell key, then u += ufl - mean */
+int zk_init_mean(const zk_init_args *a, float *mean, int nkeys, zk_stream_t s);
+
 /* ---- batches of independent simulations (batch.hip) ---- */
 #define ZK_MAX_PASSES 16
 
