@@ -566,18 +566,23 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
              * loaded and left alone): 1.98 -> 1.94 ms per 2048^2 launch. */
             const int bs = blockDim.x, last = i1 - 1;
             int i = (i0 & ~31) + threadIdx.x;
-            Part ca, cb, cc;
-            if (i < i1) {
-                ca = load_part(P, i);
+            if (i1 > i0) {
+                /* The loop runs while any lane of the wave has a slot: the
+                 * deferred deposits move data between the wave's lanes, so
+                 * no lane may leave the loop early.  Lanes past the range
+                 * load its last particle and leave it alone. */
+                Part ca, cb, cc;
+                ca = load_part(P, min(i, last));
                 cb = load_part(P, min(i + bs, last));
-            }
-            for (; i < i1; i += 3 * bs) {
-                cc = load_part(P, min(i + 2 * bs, last));
-                step(ca, i);
-                ca = load_part(P, min(i + 3 * bs, last));
-                step(cb, i + bs);
-                cb = load_part(P, min(i + 4 * bs, last));
-                step(cc, i + 2 * bs);
+                int wave0 = __builtin_amdgcn_readfirstlane(i - (int)__lane_id());
+                for (; wave0 < i1; i += 3 * bs, wave0 += 3 * bs) {
+                    cc = load_part(P, min(i + 2 * bs, last));
+                    step(ca, i);
+                    ca = load_part(P, min(i + 3 * bs, last));
+                    step(cb, i + bs);
+                    cb = load_part(P, min(i + 4 * bs, last));
+                    step(cc, i + 2 * bs);
+                }
             }
             if constexpr (FX && ZK_DEFER_CROSSING) dq.flush(lj, k);
         } else {
