@@ -93,6 +93,21 @@ void zpic_set_species_fusion(int on);
  * Batches and differentiable runs refuse such simulations. */
 void zpic_sim_set_open_y(t_simulation *sim);
 
+/* spec_deposit_charge (em2d/particles.c:1289-1324) without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the deposit is staged in a file-scope device array of
+ * the library (64 MiB per device, held until the process ends) instead of a
+ * grid allocated per call.  Same arguments and result as
+ * spec_deposit_charge (the density is added to `charge`, (nx+1) x (ny+1)
+ * floats, then folded at the periodic boundaries).  Limits: at most
+ * 8,388,608 charge nodes ((nx+1)(ny+1), e.g. 2896 x 2896 cells), and the
+ * species must already be on the device without pending host edits (after
+ * its first step); otherwise it prints why and returns -1 without touching
+ * `charge`.  Returns 0 on success.  Every call on a GPU uses that GPU's one
+ * array: calls that overlap on one GPU (from several threads) race. */
+int spec_deposit_charge_noalloc(const t_species *spec, float *charge);
+
 /* Particle initialisation of spec_new on the device, for the uniform, step
  * and slab densities (custom densities are always initialised on the
  * host): 1 on, 0 (default; environment variable ZPIC_DEVICE_INIT=1 turns

@@ -1042,6 +1042,47 @@ void spec_move_window(t_species *s)
  * diagnostics
  * ---------------------------------------------------------------------- */
 
+/* spec_deposit_charge without any allocation: the deposit and the caller's
+ * density are staged in the library's file-scope scratch array (two grids
+ * of at most ZK_NOALLOC_CHARGE_MAX nodes); the species must already be on
+ * the device with no pending host edits (an upload may allocate).  The
+ * scratch is shared by every call on a device: calls that overlap on one
+ * GPU race. */
+int spec_deposit_charge_noalloc(const t_species *cs, float *charge)
+{
+    const t_species *s = cs;
+    const struct zpic_dev_species *d = s->dev;
+    const int nrow = s->nx[0] + 1;
+    const size_t n = (size_t)nrow * (size_t)(s->nx[1] + 1);
+    if (n > (size_t)ZK_NOALLOC_CHARGE_MAX) {
+        fprintf(stderr, "spec_deposit_charge_noalloc: %zu charge nodes exceed the maximum of %d\n", n,
+                ZK_NOALLOC_CHARGE_MAX);
+        return -1;
+    }
+    if (!d || !d->ready || d->host_dirty) {
+        fprintf(stderr, "spec_deposit_charge_noalloc: species '%s' has no up-to-date device state "
+                        "(call it after a step, or use spec_deposit_charge)\n", s->name);
+        return -1;
+    }
+    float *scr = NULL;
+    const int e = zk_noalloc_scratch(&scr);
+    if (e) {
+        fprintf(stderr, "spec_deposit_charge_noalloc: no scratch on this device (HIP error %d)\n", e);
+        return -1;
+    }
+    float *rho = scr, *acc = scr + ZK_NOALLOC_CHARGE_MAX;
+    zpic_h2d_async(acc, charge, sizeof(float) * n);
+    /* the scratch keeps the previous call's contents: clear the deposit */
+    ZCHECK(zk_fill_f32(rho, n, 0.0f, zpic_stream()));
+    ZCHECK(zk_charge(&d->set[d->cur], d->np_upper, d->ctr, s->q, s->nx[0], rho, zpic_stream()));
+    ZCHECK(zk_add_f32(acc, rho, n, zpic_stream()));
+    zpic_d2h(charge, acc, sizeof(float) * n);
+    if (!s->moving_window)
+        for (int j = 0; j < s->nx[1] + 1; j++) charge[j * nrow] += charge[s->nx[0] + j * nrow];
+    if (!d->open_y)
+        for (int i = 0; i < s->nx[0] + 1; i++) charge[i] += charge[i + s->nx[1] * nrow];
+    return 0;
+}
 void spec_deposit_charge(const t_species *cs, float *charge)
 {
     /* CIC deposit on the device, boundary fold on the host

@@ -17,9 +17,16 @@
  * zk_charge deposits the CIC charge density (spec_deposit_charge,
  * em2d/particles.c:1289-1324, before its boundary fold).
  */
+#include <mutex>
+
 #include "fields_core.h"
 
 using namespace zk;
+
+/* Scratch of spec_deposit_charge_noalloc: the deposit (first half) and the
+ * caller's density (second half), ZK_NOALLOC_CHARGE_MAX nodes each.  Every
+ * device holds its own copy for the life of the process. */
+static __device__ __align__(256) float scratch[2 * ZK_NOALLOC_CHARGE_MAX];
 
 namespace {
 
@@ -163,7 +170,46 @@ __global__ void pha_kernel(zk_parts p, const zk_spec_ctr *ctr, int qa, int qb, f
     }
 }
 
+__global__ void add_kernel(float *acc, const float *v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc[i] += v[i];
+}
+
+#define ZK_MAX_DEVICES 64
+float *g_scratch[ZK_MAX_DEVICES];
+std::mutex g_scratch_lock;
+
 } // namespace
+
+/* The array's name is not a device address in host code: its address on a
+ * device comes from hipGetSymbolAddress, called once per device with that
+ * device current, and only that address is passed to kernels and copies. */
+extern "C" int zk_noalloc_scratch(float **ptr)
+{
+    int dev = 0;
+    int e = (int)hipGetDevice(&dev);
+    if (e) return e;
+    if (dev < 0 || dev >= ZK_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = (int)hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e) return e;
+        g_scratch[dev] = (float *)p;
+    }
+    *ptr = g_scratch[dev];
+    return 0;
+}
+
+extern "C" int zk_add_f32(float *acc, const float *v, size_t n, zk_stream_t s)
+{
+    if (n == 0) return 0;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(add_kernel, dim3((unsigned)blocks), dim3(256), 0, s, acc, v, n);
+    ZK_LAUNCH_CHECK();
+}
 
 extern "C" int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, int qa, int qb, float dx0, float dx1,
                       int na, int nb, float amin, float bmin, float ra, float rb, float q, float *out, zk_stream_t s)

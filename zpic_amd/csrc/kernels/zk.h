@@ -158,6 +158,13 @@ int zk_emf_energy(const float *E, const float *B, int nx, int ny, int nrow,
                          double *partial, int nblk, double *out6, double scale, zk_stream_t s);
 int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, float q,
                      int nx, float *rho, zk_stream_t s);
+/* Scratch of the allocation-free charge deposit: a file-scope device array
+ * of 2 * ZK_NOALLOC_CHARGE_MAX floats (64 MiB) on every device the library
+ * runs on.  zk_noalloc_scratch returns its address on the current device
+ * (looked up once per device).  zk_add_f32: acc[k] += v[k]. */
+#define ZK_NOALLOC_CHARGE_MAX (8 << 20)
+int zk_noalloc_scratch(float **ptr);
+int zk_add_f32(float *acc, const float *v, size_t n, zk_stream_t s);
 /* out (na x nb, row length na) += phase-space density of quantities qa, qb
  * (em2d/particles.c:1569-1632 weights): amin/bmin the lower range bounds,
  * ra/rb the bins per unit */
