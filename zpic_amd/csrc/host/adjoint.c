@@ -64,11 +64,11 @@ struct zpic_diff {
 
 #define CS(D, c, k) (&(D)->cs[(size_t)(c) * (D)->ns + (k)])
 
-static void parts_alloc(zk_parts *p, int n, int ids)
+static void parts_alloc(zk_parts *p, int n, int ids, int cbits)
 {
     const size_t m = (size_t)(n > 0 ? n : 1);
-    p->ix = zpic_dmalloc(4 * m);
-    p->iy = zpic_dmalloc(4 * m);
+    p->cell = zpic_dmalloc(4 * m);
+    p->cbits = cbits;
     p->x = zpic_dmalloc(4 * m);
     p->y = zpic_dmalloc(4 * m);
     p->ux = zpic_dmalloc(4 * m);
@@ -79,8 +79,7 @@ static void parts_alloc(zk_parts *p, int n, int ids)
 
 static void parts_free(zk_parts *p)
 {
-    zpic_dfree(p->ix);
-    zpic_dfree(p->iy);
+    zpic_dfree(p->cell);
     zpic_dfree(p->x);
     zpic_dfree(p->y);
     zpic_dfree(p->ux);
@@ -131,7 +130,8 @@ static void records_alloc(zpic_diff *D, int nrec)
     D->nrec = nrec;
     D->rp = calloc((size_t)nrec * (D->ns > 0 ? D->ns : 1), sizeof(zk_parts));
     for (int n = 0; n < nrec; n++)
-        for (int k = 0; k < D->ns; k++) parts_alloc(&D->rp[(size_t)n * D->ns + k], D->np[k], 0);
+        for (int k = 0; k < D->ns; k++)
+            parts_alloc(&D->rp[(size_t)n * D->ns + k], D->np[k], 0, D->sim->species[k].dev->cbits);
     D->rE = calloc((size_t)nrec + 1, sizeof(float *));
     D->rB = calloc((size_t)nrec + 1, sizeof(float *));
     for (int n = 0; n <= nrec; n++) {
@@ -210,7 +210,7 @@ zpic_diff *zpic_diff_new(t_simulation *sim, int nsteps, int ckpt, int loss_first
         D->cE[c] = zpic_dmalloc(gb);
         D->cB[c] = zpic_dmalloc(gb);
         for (int k = 0; k < D->ns; k++) {
-            parts_alloc(&CS(D, c, k)->p, D->np[k], 1);
+            parts_alloc(&CS(D, c, k)->p, D->np[k], 1, D->sim->species[k].dev->cbits);
             CS(D, c, k)->tile_off = zpic_dmalloc(sizeof(int) * (size_t)(t->ntiles + 2));
         }
     }

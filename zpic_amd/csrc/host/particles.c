@@ -310,8 +310,8 @@ static void device_init(t_species *s, const int r[2][2])
         a.col_count = dtab + 2 * perrow + ncols;
         a.lat = dlat;
         const zk_parts *P = &d->set[d->cur];
-        a.ix = P->ix;
-        a.iy = P->iy;
+        a.cell = P->cell;
+        a.cbits = P->cbits;
         a.x = P->x;
         a.y = P->y;
         a.ux = P->ux;
@@ -456,6 +456,7 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
     const int r[2][2] = {{0, nx[0] - 1}, {0, nx[1] - 1}};
     struct zpic_dev_species *d = calloc(1, sizeof(*d));
     s->dev = d;
+    d->cbits = zpic_cell_bits(nx[0], nx[1]);
     const int dens = s->density.type;
     if (zpic_device_init() && (dens == UNIFORM || dens == STEP || dens == SLAB)) {
         device_init(s, r);
@@ -485,8 +486,7 @@ static void species_free_device(struct zpic_dev_species *d)
     }
     for (int k = 0; k < 2; k++) {
         zk_parts *p = &d->set[k];
-        zpic_dfree(p->ix);
-        zpic_dfree(p->iy);
+        zpic_dfree(p->cell);
         zpic_dfree(p->x);
         zpic_dfree(p->y);
         zpic_dfree(p->ux);
@@ -574,10 +574,10 @@ double spec_perf(void)
 /* ------------------------------------------------------------------------
  * device residency
  * ---------------------------------------------------------------------- */
-static void alloc_set(zk_parts *p, int cap, int ids)
+static void alloc_set(zk_parts *p, int cap, int ids, int cbits)
 {
-    p->ix = zpic_dmalloc(sizeof(int) * (size_t)cap);
-    p->iy = zpic_dmalloc(sizeof(int) * (size_t)cap);
+    p->cell = zpic_dmalloc(sizeof(int) * (size_t)cap);
+    p->cbits = cbits;
     p->x = zpic_dmalloc(sizeof(float) * (size_t)cap);
     p->y = zpic_dmalloc(sizeof(float) * (size_t)cap);
     p->ux = zpic_dmalloc(sizeof(float) * (size_t)cap);
@@ -606,11 +606,10 @@ static void grow_sets(struct zpic_dev_species *d, int need)
     zpic_sync();
     for (int k = 0; k < 2; k++) {
         zk_parts old = d->set[k], nw;
-        alloc_set(&nw, cap, d->track_ids);
-        if (k == d->cur && old.ix && d->np_upper > 0) {
+        alloc_set(&nw, cap, d->track_ids, d->cbits);
+        if (k == d->cur && old.cell && d->np_upper > 0) {
             const size_t n = (size_t)d->np_upper;
-            zpic_d2d_async(nw.ix, old.ix, 4 * n);
-            zpic_d2d_async(nw.iy, old.iy, 4 * n);
+            zpic_d2d_async(nw.cell, old.cell, 4 * n);
             zpic_d2d_async(nw.x, old.x, 4 * n);
             zpic_d2d_async(nw.y, old.y, 4 * n);
             zpic_d2d_async(nw.ux, old.ux, 4 * n);
@@ -620,8 +619,7 @@ static void grow_sets(struct zpic_dev_species *d, int need)
                 zpic_d2d_async(nw.id, old.id, 8 * n);
         }
         zpic_sync();
-        zpic_dfree(old.ix);
-        zpic_dfree(old.iy);
+        zpic_dfree(old.cell);
         zpic_dfree(old.x);
         zpic_dfree(old.y);
         zpic_dfree(old.ux);

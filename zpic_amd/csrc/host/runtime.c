@@ -64,6 +64,17 @@ int zpic_species_fusion(void)
     return g_fuse_species;
 }
 
+/* Bits of the x cell index in the packed particle cell (ix | iy << bits):
+ * the least b with nx <= 2^b; the pair must fit 31 bits. */
+int zpic_cell_bits(int nx, int ny)
+{
+    int bx = 0, by = 0;
+    while ((1 << bx) < nx) bx++;
+    while ((1 << by) < ny) by++;
+    if (bx + by > 31) zpic_fatal("grid %d x %d has too many cells for 31-bit cell indices", nx, ny);
+    return bx;
+}
+
 int zpic_fixed_point_exp(float q)
 {
     /* Q = 2^k, the power of two at or above |q| */

@@ -72,6 +72,7 @@ struct zpic_dev_species {
     t_species *owner;
     int pref[3];          /* preferred tile (tx, ty, halo), 0: process-wide setting */
     int open_y;           /* particles leaving through y are removed */
+    int cbits;            /* bits of ix in the packed cell of the device sets */
     int dev_unsorted;     /* particles made on the device by spec_new, in set[cur],
                              not yet sorted into tiles (no tiles chosen yet) */
 };

@@ -52,6 +52,7 @@ int zpic_species_fusion(void);
 /* fixed-point deposit constants for particle charge q (see push_core.h) */
 void zpic_fixed_point_scale(float q, float *scale, float *unscale);
 int zpic_fixed_point_exp(float q); /* k with Q = 2^k the power of two at or above |q| */
+int zpic_cell_bits(int nx, int ny); /* bits of ix in the packed particle cell */
 size_t zpic_grid_nodes(const int nx[2]); /* (nx+3)*(ny+3) guard-inclusive nodes */
 
 /* GPU timers: spec_time() / emf_time(), and per-kernel-class timers that

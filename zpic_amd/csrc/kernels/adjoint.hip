@@ -642,7 +642,7 @@ __device__ inline float adj_share(const zk_adj_push_args &a, const S &src, int i
     const int lane = __lane_id();
     float mx = 0.0f;
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const int ix = a.p.ix[i], iy = a.p.iy[i];
+        const int ix = zk_cell_ix(a.p.cell[i], a.p.cbits), iy = zk_cell_iy(a.p.cell[i], a.p.cbits);
         const int lx = ix - tx0, ly = iy - ty0;
         const bool inside = lx >= -t.H && lx < t.TX + t.H && ly >= -t.H && ly < t.TY + t.H;
         const unsigned long long out = __ballot(!inside);
@@ -751,8 +751,9 @@ __global__ void __launch_bounds__(256) adj_tile_kernel(zk_adj_push_args a)
         const int i = a.ovf_list[i0 + q];
         float l[5];
         load_adj(a, i, l);
-        const float m =
-            adjoint_one(gsrc, a.p.ix[i] + 1, a.p.iy[i] + 1, a.p.x[i], a.p.y[i], a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
+        const int c = a.p.cell[i];
+        const float m = adjoint_one(gsrc, zk_cell_ix(c, a.p.cbits) + 1, zk_cell_iy(c, a.p.cbits) + 1, a.p.x[i], a.p.y[i],
+                                    a.p.ux[i], a.p.uy[i], a.p.uz[i], l, k);
         if (m > 0.0f) atomicMax(a.amax, __float_as_uint(m));
         store_adj(a, i, l);
     }
@@ -828,8 +829,7 @@ __global__ void adj_by_id_kernel(zk_adj_vec v, const int64_t *__restrict__ ids, 
 __global__ void copy_parts_kernel(zk_parts dst, zk_parts src, int n)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        dst.ix[i] = src.ix[i];
-        dst.iy[i] = src.iy[i];
+        dst.cell[i] = src.cell[i];
         dst.x[i] = src.x[i];
         dst.y[i] = src.y[i];
         dst.ux[i] = src.ux[i];

@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(ZK_SORT_THREADS) count_batch(const zk_bspec *s
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     if (!sorts_now(S)) return;
     const zk_parts &p = S.set[S.pcur];
-    count_stay_body(p.ix, p.iy, S.offs[S.ocur], S.stay, S.cnt, S.nbr, S.tailn, S.a.t, items[2 * blockIdx.x + 1],
+    count_stay_body(p.cell, p.cbits, S.offs[S.ocur], S.stay, S.cnt, S.nbr, S.tailn, S.a.t, items[2 * blockIdx.x + 1],
                     S.a.t.ntiles);
 }
 

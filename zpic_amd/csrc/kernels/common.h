@@ -44,6 +44,12 @@ __device__ inline unsigned long long zk_fx(float v, float scale)
     return (unsigned long long)__double_as_longlong(r) - 0x4338000000000000ull;
 }
 
+/* Packed cell index of the particle arrays (zk_parts): ix | iy << cbits,
+ * -1 for a removed particle, whose ix reads as ZK_DEAD. */
+__device__ inline int zk_cell_ix(int c, int cbits) { return c < 0 ? ZK_DEAD : (c & ((1 << cbits) - 1)); }
+__device__ inline int zk_cell_iy(int c, int cbits) { return c >> cbits; }
+__device__ inline int zk_cell(int ix, int iy, int cbits) { return ix == ZK_DEAD ? -1 : (ix | (iy << cbits)); }
+
 /* Floor modulo for periodic wrap. */
 __device__ inline int zk_wrap(int i, int n)
 {

@@ -107,9 +107,9 @@ __global__ void charge_kernel(zk_parts p, const zk_spec_ctr *ctr, float q, int n
     const int n = ctr->np_total;
     const int nrow = nx + 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int ix = p.ix[i];
-        if (ix == ZK_DEAD) continue;
-        const int c = ix + nrow * p.iy[i];
+        const int cell = p.cell[i];
+        if (cell < 0) continue;
+        const int c = zk_cell_ix(cell, p.cbits) + nrow * zk_cell_iy(cell, p.cbits);
         const float wx = p.x[i], wy = p.y[i];
         unsafeAtomicAdd(&rho[c], (1.0f - wx) * (1.0f - wy) * q);
         unsafeAtomicAdd(&rho[c + 1], (wx) * (1.0f - wy) * q);
@@ -123,8 +123,8 @@ __global__ void charge_kernel(zk_parts p, const zk_spec_ctr *ctr, float q, int n
 __device__ inline float pha_quantity(const zk_parts &p, int i, int q, float dx0, float dx1)
 {
     switch (q) {
-    case 1: return (p.x[i] + p.ix[i]) * dx0;
-    case 2: return (p.y[i] + p.iy[i]) * dx1;
+    case 1: return (p.x[i] + zk_cell_ix(p.cell[i], p.cbits)) * dx0;
+    case 2: return (p.y[i] + zk_cell_iy(p.cell[i], p.cbits)) * dx1;
     case 4: return p.ux[i];
     case 5: return p.uy[i];
     case 6: return p.uz[i];
@@ -147,7 +147,7 @@ __global__ void pha_kernel(zk_parts p, const zk_spec_ctr *ctr, int qa, int qb, f
     }
     const int n = ctr->np_total;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (p.ix[i] == ZK_DEAD) continue;
+        if (p.cell[i] < 0) continue;
         const float va = (pha_quantity(p, i, qa, dx0, dx1) - amin) * ra;
         const float vb = (pha_quantity(p, i, qb, dx0, dx1) - bmin) * rb;
         const int ia = (int)(va + 0.5f), ib = (int)(vb + 0.5f);

@@ -147,8 +147,7 @@ __global__ void positions_kernel(zk_init_args a)
         const long long j = p / a.perrow;
         const int e = (int)(p - j * a.perrow);
         const int i = a.tab_i[e], k = a.tab_k[e];
-        a.ix[p] = i;
-        a.iy[p] = a.row0 + (int)j;
+        a.cell[p] = zk_cell(i, a.row0 + (int)j, a.cbits);
         a.x[p] = a.lat[2 * k];
         a.y[p] = a.lat[2 * k + 1];
     }
@@ -195,7 +194,7 @@ __global__ void apply_mean_kernel(zk_init_args a, const float *mean)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < a.np;
          p += (long long)gridDim.x * blockDim.x) {
-        const int idx = a.ix[p] + a.stride * a.iy[p];
+        const int idx = zk_cell_ix(a.cell[p], a.cbits) + a.stride * zk_cell_iy(a.cell[p], a.cbits);
         a.ux[p] += a.ufl[0] - mean[3 * idx + 0];
         a.uy[p] += a.ufl[1] - mean[3 * idx + 1];
         a.uz[p] += a.ufl[2] - mean[3 * idx + 2];

@@ -304,8 +304,9 @@ __device__ inline Part load_part(const zk_parts &P, int i)
 {
     const unsigned o = (unsigned)i * 4u;
     Part c;
-    c.ix = zk_at(P.ix, o);
-    c.iy = zk_at(P.iy, o);
+    const int cell = zk_at(P.cell, o);
+    c.ix = zk_cell_ix(cell, P.cbits);
+    c.iy = zk_cell_iy(cell, P.cbits);
     c.x = zk_at(P.x, o);
     c.y = zk_at(P.y, o);
     c.ux = zk_at(P.ux, o);
@@ -317,8 +318,7 @@ __device__ inline Part load_part(const zk_parts &P, int i)
 __device__ inline void store_part(const zk_parts &P, int i, const Part &c)
 {
     const unsigned o = (unsigned)i * 4u;
-    zk_at(P.ix, o) = c.ix;
-    zk_at(P.iy, o) = c.iy;
+    zk_at(P.cell, o) = zk_cell(c.ix, c.iy, P.cbits);
     zk_at(P.x, o) = c.x;
     zk_at(P.y, o) = c.y;
     zk_at(P.ux, o) = c.ux;

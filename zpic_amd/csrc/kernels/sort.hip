@@ -27,10 +27,9 @@ using namespace zk;
 
 namespace {
 
-__global__ void count_kernel(const int *__restrict__ ix, const int *__restrict__ iy, const zk_spec_ctr *ctr,
-                             int *cnt, zk_tiles t)
+__global__ void count_kernel(const int *__restrict__ cell, int cbits, const zk_spec_ctr *ctr, int *cnt, zk_tiles t)
 {
-    count_body(ix, iy, ctr, cnt, t, blockIdx.x, gridDim.x);
+    count_body(cell, cbits, ctr, cnt, t, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) scan_kernel(int *cnt, int *off, int *cursor, zk_spec_ctr *ctr, int ntiles)
@@ -50,10 +49,10 @@ __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, i
 }
 
 __global__ void __launch_bounds__(ZK_SORT_THREADS)
-    count_stay_kernel(const int *__restrict__ ix, const int *__restrict__ iy, const int *old_off, int *stay, int *move,
-                      int *nbr, int *tailn, zk_tiles t)
+    count_stay_kernel(const int *__restrict__ cell, int cbits, const int *old_off, int *stay, int *move, int *nbr,
+                      int *tailn, zk_tiles t)
 {
-    count_stay_body(ix, iy, old_off, stay, move, nbr, tailn, t, blockIdx.x, gridDim.x);
+    count_stay_body(cell, cbits, old_off, stay, move, nbr, tailn, t, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) scan_stay_kernel(const int *stay, int *move, int *tailn, int *new_off,
@@ -69,7 +68,8 @@ __global__ void __launch_bounds__(ZK_SORT_THREADS)
     scatter_stay_body(src, dst, old_off, new_off, stay, nbr, cursor, t, blockIdx.x, gridDim.x);
 }
 
-/* t_part AoS (28 bytes: ix, iy, x, y, ux, uy, uz) <-> SoA */
+/* t_part AoS (28 bytes: ix, iy, x, y, ux, uy, uz) <-> SoA with the cell
+ * index pair packed */
 __global__ void unpack_kernel(const int *__restrict__ aos, int n, zk_parts dst, int64_t id0, zk_spec_ctr *ctr,
                               int append)
 {
@@ -77,8 +77,7 @@ __global__ void unpack_kernel(const int *__restrict__ aos, int n, zk_parts dst, 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int *r = aos + 7 * (size_t)i;
         const int d = base + i;
-        dst.ix[d] = r[0];
-        dst.iy[d] = r[1];
+        dst.cell[d] = zk_cell(r[0], r[1], dst.cbits);
         dst.x[d] = __int_as_float(r[2]);
         dst.y[d] = __int_as_float(r[3]);
         dst.ux[d] = __int_as_float(r[4]);
@@ -103,8 +102,9 @@ __global__ void pack_kernel(zk_parts src, int n, int *__restrict__ aos)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int *r = aos + 7 * (size_t)i;
-        r[0] = src.ix[i];
-        r[1] = src.iy[i];
+        const int c = src.cell[i];
+        r[0] = zk_cell_ix(c, src.cbits);
+        r[1] = zk_cell_iy(c, src.cbits);
         r[2] = __float_as_int(src.x[i]);
         r[3] = __float_as_int(src.y[i]);
         r[4] = __float_as_int(src.ux[i]);
@@ -130,7 +130,7 @@ extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ct
     /* the slot count in use before the scan is kept in off[ntiles + 1] */
     int *n_old = off + t->ntiles + 1;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, n_old);
-    hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->ix, src->iy, ctr, cnt, *t);
+    hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->cell, src->cbits, ctr, cnt, *t);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles);
     hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t, spread);
     ZK_LAUNCH_CHECK();
@@ -148,7 +148,7 @@ extern "C" int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_c
     const unsigned tb = tail_upper > 0 ? grid_for(tail_upper, 1024, 4096) : 0;
     const unsigned g = (unsigned)t->ntiles + tb;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, old_off + t->ntiles + 1);
-    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, src->ix, src->iy, old_off, stay, move,
+    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, src->cell, src->cbits, old_off, stay, move,
                        nbr, tailn, *t);
     hipLaunchKernelGGL(scan_stay_kernel, dim3(1), dim3(1024), 0, s, stay, move, tailn, new_off, cursor, ctr,
                        t->ntiles);

@@ -21,14 +21,20 @@ __device__ inline int tile_key(int ix, int iy, const zk_tiles &t)
     return zk_div(ix, 1.0f / t.TX) + zk_div(iy, 1.0f / t.TY) * t.ntx;
 }
 
-__device__ inline void count_body(const int *__restrict__ ix, const int *__restrict__ iy, const zk_spec_ctr *ctr,
-                                  int *cnt, const zk_tiles &t, const int block, const int nblocks)
+/* tile of a packed cell (zk_parts), -1 for a removed particle */
+__device__ inline int tile_key_cell(int c, int cbits, const zk_tiles &t)
+{
+    return tile_key(zk_cell_ix(c, cbits), zk_cell_iy(c, cbits), t);
+}
+
+__device__ inline void count_body(const int *__restrict__ cell, int cbits, const zk_spec_ctr *ctr, int *cnt,
+                                  const zk_tiles &t, const int block, const int nblocks)
 {
     const int n = ctr->np_total;
     const int stride = nblocks * blockDim.x;
     for (int base = block * blockDim.x; base < n; base += stride) {
         const int i = base + threadIdx.x;
-        const int key = (i < n) ? tile_key(ix[i], iy[i], t) : -1;
+        const int key = (i < n) ? tile_key_cell(cell[i], cbits, t) : -1;
         zk_run_reserve(key, cnt);
     }
 }
@@ -89,11 +95,10 @@ __device__ inline void scatter_body(const zk_parts &src, const zk_parts &dst, co
     const int stride = nblocks * blockDim.x;
     for (int base = block * blockDim.x; base < n; base += stride) {
         const int i = base + threadIdx.x;
-        int ix = 0, iy = 0, key = -1;
+        int c = -1, key = -1;
         if (i < n) {
-            ix = src.ix[i];
-            iy = src.iy[i];
-            key = tile_key(ix, iy, t);
+            c = src.cell[i];
+            key = tile_key_cell(c, src.cbits, t);
         }
         int d = zk_run_reserve(key, cursor);
         if (key >= 0) {
@@ -101,8 +106,7 @@ __device__ inline void scatter_body(const zk_parts &src, const zk_parts &dst, co
                 const int base = off[key];
                 d = base + interleave(d - base, off[key + 1] - base);
             }
-            dst.ix[d] = ix;
-            dst.iy[d] = iy;
+            dst.cell[d] = c;
             dst.x[d] = src.x[i];
             dst.y[d] = src.y[i];
             dst.ux[d] = src.ux[i];
@@ -204,9 +208,9 @@ __device__ inline int wave_slot_add(int slot, int *lcnt)
  * reserving their runs (nbr, move); farther movers count in tailn.  Blocks
  * beyond count chunks of the tail into tailn.  old_off[ntiles] is the end
  * of the tile ranges, old_off[ntiles + 1] the slot count in use. */
-__device__ inline void count_stay_body(const int *__restrict__ ix, const int *__restrict__ iy,
-                                       const int *__restrict__ old_off, int *stay, int *move, int *nbr, int *tailn,
-                                       const zk_tiles &t, const int block, const int nblocks)
+__device__ inline void count_stay_body(const int *__restrict__ cell, int cbits, const int *__restrict__ old_off,
+                                       int *stay, int *move, int *nbr, int *tailn, const zk_tiles &t,
+                                       const int block, const int nblocks)
 {
     __shared__ int lc[9], wst[ZK_SORT_THREADS / ZK_WAVE];
     const int lane = __lane_id();
@@ -218,16 +222,15 @@ __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__
         wave_piece(old_off[block], old_off[block + 1], w, nw, &wlo, &whi);
         int nstay = 0;
         for (int base = wlo & ~31; base < whi; base += ZK_SORT_UNROLL * ZK_WAVE) {
-            int cx[ZK_SORT_UNROLL], cy[ZK_SORT_UNROLL];
+            int cc[ZK_SORT_UNROLL];
 #pragma unroll
             for (int u = 0; u < ZK_SORT_UNROLL; u++) {
                 const int i = base + u * ZK_WAVE + lane;
-                cx[u] = (i >= wlo && i < whi) ? ix[i] : ZK_DEAD;
-                cy[u] = (i >= wlo && i < whi) ? iy[i] : 0;
+                cc[u] = (i >= wlo && i < whi) ? cell[i] : -1;
             }
 #pragma unroll
             for (int u = 0; u < ZK_SORT_UNROLL; u++) {
-                const int key = tile_key(cx[u], cy[u], t);
+                const int key = tile_key_cell(cc[u], cbits, t);
                 const bool st = key == block;
                 nstay += __popcll(__ballot(st));
                 const int slot = (key >= 0 && !st) ? nbr_slot(key, block, t) : -1;
@@ -254,7 +257,7 @@ __device__ inline void count_stay_body(const int *__restrict__ ix, const int *__
         const int stride = (nblocks - t.ntiles) * blockDim.x;
         for (int base = n0 + (block - t.ntiles) * blockDim.x; base < n1; base += stride) {
             const int i = base + threadIdx.x;
-            const int key = (i < n1) ? tile_key(ix[i], iy[i], t) : -1;
+            const int key = (i < n1) ? tile_key_cell(cell[i], cbits, t) : -1;
             zk_run_reserve(key, tailn);
         }
     }
@@ -315,10 +318,9 @@ __device__ inline void scan_stay_body(const int *stay, int *move, int *tailn, in
     }
 }
 
-__device__ inline void copy_part(const zk_parts &src, const zk_parts &dst, int i, int d, int ix, int iy)
+__device__ inline void copy_part(const zk_parts &src, const zk_parts &dst, int i, int d, int c)
 {
-    dst.ix[d] = ix;
-    dst.iy[d] = iy;
+    dst.cell[d] = c;
     dst.x[d] = src.x[i];
     dst.y[d] = src.y[i];
     dst.ux[d] = src.ux[i];
@@ -353,7 +355,7 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
         __syncthreads();
         const unsigned long long lt = (1ull << lane) - 1ull;
         for (int base = wlo & ~31; base < whi; base += ZK_SORT_UNROLL * ZK_WAVE) {
-            int cx[ZK_SORT_UNROLL], cy[ZK_SORT_UNROLL];
+            int cc[ZK_SORT_UNROLL];
             float x[ZK_SORT_UNROLL], y[ZK_SORT_UNROLL], ux[ZK_SORT_UNROLL], uy[ZK_SORT_UNROLL], uz[ZK_SORT_UNROLL];
             int64_t id[ZK_SORT_UNROLL];
 #pragma unroll
@@ -361,8 +363,7 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
                 const int i = base + u * ZK_WAVE + lane;
                 const bool in = i >= wlo && i < whi;
                 const unsigned o = (unsigned)i * 4u;
-                cx[u] = in ? zk_at(src.ix, o) : ZK_DEAD;
-                cy[u] = in ? zk_at(src.iy, o) : 0;
+                cc[u] = in ? zk_at(src.cell, o) : -1;
                 x[u] = in ? zk_at(src.x, o) : 0.f;
                 y[u] = in ? zk_at(src.y, o) : 0.f;
                 ux[u] = in ? zk_at(src.ux, o) : 0.f;
@@ -372,7 +373,7 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
             }
 #pragma unroll
             for (int u = 0; u < ZK_SORT_UNROLL; u++) {
-                const int key = tile_key(cx[u], cy[u], t);
+                const int key = tile_key_cell(cc[u], src.cbits, t);
                 const bool st = key == block;
                 const unsigned long long ms = __ballot(st);
                 int d = placed + __popcll(ms & lt);
@@ -385,8 +386,7 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
                 if (far) d = dm;
                 if (key >= 0) {
                     const unsigned od = (unsigned)d * 4u;
-                    zk_at(dst.ix, od) = cx[u];
-                    zk_at(dst.iy, od) = cy[u];
+                    zk_at(dst.cell, od) = cc[u];
                     zk_at(dst.x, od) = x[u];
                     zk_at(dst.y, od) = y[u];
                     zk_at(dst.ux, od) = ux[u];
@@ -401,14 +401,13 @@ __device__ inline void scatter_stay_body(const zk_parts &src, const zk_parts &ds
         const int stride = (nblocks - t.ntiles) * blockDim.x;
         for (int base = n0 + (block - t.ntiles) * blockDim.x; base < n1; base += stride) {
             const int i = base + threadIdx.x;
-            int ix = 0, iy = 0, key = -1;
+            int c = -1, key = -1;
             if (i < n1) {
-                ix = src.ix[i];
-                iy = src.iy[i];
-                key = tile_key(ix, iy, t);
+                c = src.cell[i];
+                key = tile_key_cell(c, src.cbits, t);
             }
             const int dm = zk_run_reserve(key, cursor);
-            if (key >= 0) copy_part(src, dst, i, dm, ix, iy);
+            if (key >= 0) copy_part(src, dst, i, dm, c);
         }
     }
 }

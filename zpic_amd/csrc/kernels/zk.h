@@ -8,8 +8,8 @@
  * Grids are float3 arrays of (nx+3) x (ny+3) nodes including guard cells
  * (one lower, two upper guards per axis), row stride nrow = nx + 3, and are
  * addressed from the buffer origin: node (i, j), i,j >= -1, sits at
- * (i + 1) + (j + 1) * nrow.  Particles are stored as seven separate arrays
- * (ix, iy, x, y, ux, uy, uz), grouped by tile.
+ * (i + 1) + (j + 1) * nrow.  Particles are stored as six separate arrays
+ * (packed cell index, x, y, ux, uy, uz), grouped by tile.
  */
 #ifndef ZK_H
 #define ZK_H
@@ -27,7 +27,8 @@ typedef struct ihipStream_t *zk_stream_t;
 extern "C" {
 #endif
 
-/* Cell index that marks a removed particle (absorbing boundary). */
+/* Cell index ix that marks a removed particle (absorbing boundary) in the
+ * kernels' registers; in memory a removed particle's packed cell is -1. */
 #define ZK_DEAD (-0x40000000)
 
 /* Most particle slots of one species: the push addresses the particle
@@ -44,15 +45,19 @@ typedef struct zk_spec_ctr {
     double energy;              /* kinetic energy of the last push            */
 } zk_spec_ctr;
 
-/* Particle arrays of one buffer set. */
+/* Particle arrays of one buffer set: the cell index pair is packed into
+ * one word, cell = ix | iy << cbits (nx <= 2^cbits, cbits + bits of ny <=
+ * 31), -1 for a removed particle; 24 bytes per particle. */
 typedef struct zk_parts {
-    int *ix, *iy;
+    int *cell;
     float *x, *y, *ux, *uy, *uz;
     int64_t *id;                /* optional identities (NULL when untracked) */
+    int cbits;
 } zk_parts;
 
 /* Tile geometry shared by the push, the sort and the current reduction.
- * Tiles are TX x TY cells; nx % TX == 0, ny % TY == 0.  A tile's window is
+ * Tiles are TX x TY cells (the last tile of a row or column may be
+ * narrower: ntx = ceil(nx / TX)).  A tile's window is
  * its cells plus H cells of drift margin plus the interpolation/deposit
  * stencil (1 below, 2 above): WX = TX + 2H + 3 nodes, WY likewise. */
 typedef struct zk_tiles {
@@ -179,7 +184,8 @@ typedef struct zk_init_args {
     const int *tab_i, *tab_k;           /* [perrow] column and lattice point of a row's particles */
     const int *col_start, *col_count;   /* [ncols] first entry and entries of a column in a row */
     const float *lat;                   /* [2 ppc] lattice points in the cell */
-    int *ix, *iy;                       /* outputs, particle order */
+    int *cell;                          /* outputs, particle order (cell packed, cbits) */
+    int cbits;
     float *x, *y, *ux, *uy, *uz;
     int stride;                         /* cell key ix + stride iy of the mean removal (nx[1]) */
     float uth[3], ufl[3];
