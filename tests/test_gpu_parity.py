@@ -296,6 +296,15 @@ def test_gpu_smoothing_matches_filter_model(gpu):
         xp = [(0.25, 0.5)] * n + [(-1.0 / t, ((4.0 + 2.0 * n) / n) / t)]
         exp = filter_model(Jr, xp, [(0.25, 0.5)] * n)
         assert np.abs(Js - exp)[1:49, 1:65].max() <= 2e-6 * np.abs(Jr).max()
+        # the deposit is order-free, so both runs fold the same J; the passes
+        # are then the FP32 expression of the reference bit for bit, with the
+        # compensator's weights as the host computes them (float b, total)
+        f32 = np.float32
+        wb = f32((4.0 + 2.0 * n) / n)
+        tot = f32(2) * f32(-1) + wb
+        xp32 = [(0.25, 0.5)] * n + [(f32(-1) / tot, wb / tot)]
+        exp32 = filter_model(a.state()["J"], xp32, [(0.25, 0.5)] * n, dtype=f32)
+        assert Js.astype(f32)[1:49, 1:65].tobytes() == exp32[1:49, 1:65].tobytes()
     finally:
         a.close()
         b.close()

@@ -230,12 +230,17 @@ def _binomial(J, axis, passes, sa=0.25, sb=0.5):
     return J
 
 
-def filter_model(J, xpasses, ypasses):
+def filter_model(J, xpasses, ypasses, dtype=np.float64):
     """Filter passes on a guard-inclusive periodic grid J[rows, cols, 3]
     (one lower, two upper guards) with the guard rules of kernel_x/kernel_y
     (current.c:316-413): an x pass refreshes the x guards of interior rows
-    only, so the guard rows a following y pass reads are not x-filtered."""
-    J = J.astype(np.float64).copy()
+    only, so the guard rows a following y pass reads are not x-filtered.
+    With dtype float32 every product and sum is rounded to FP32 in the
+    order sa*l + sb*m + sa*u, the reference's expression without
+    contraction."""
+    J = J.astype(dtype).copy()
+    xpasses = [(dtype(sa), dtype(sb)) for sa, sb in xpasses]
+    ypasses = [(dtype(sa), dtype(sb)) for sa, sb in ypasses]
     ny, nx = J.shape[0] - 3, J.shape[1] - 3
     for sa, sb in xpasses:
         I = J[1:1 + ny]
@@ -270,6 +275,9 @@ def test_smoothing_matches_three_point_filter_model(oracle):
             Js = b.state()["J"].astype(np.float64)
             exp = filter_model(Jr, [(0.25, 0.5)] * xp, [(0.25, 0.5)] * yp)
             assert np.abs(Js - exp)[1:25, 1:33].max() <= 1e-6 * np.abs(Jr).max()
+            # in FP32, operation by operation, the model is the oracle bit for bit
+            exp32 = filter_model(a.state()["J"], [(0.25, 0.5)] * xp, [(0.25, 0.5)] * yp, dtype=np.float32)
+            assert b.state()["J"][1:25, 1:33].tobytes() == exp32[1:25, 1:33].tobytes()
             if yp == 0:
                 conv = _binomial(Jr[1:25, 1:33], 1, xp)
                 assert np.abs(Js[1:25, 1:33] - conv).max() <= 1e-6 * np.abs(Jr).max()

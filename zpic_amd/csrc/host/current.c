@@ -12,7 +12,13 @@
 #include <string.h>
 
 #include "zpic_internal.h"
+
 #include "zpic_zdf.h"
+
+/* 0: one launch per filter pass (for comparisons) */
+#ifndef ZPIC_SMOOTH_X_FUSE
+#define ZPIC_SMOOTH_X_FUSE 1
+#endif
 
 void current_new(t_current *c, int nx[], float box[], float dt)
 {
@@ -175,9 +181,20 @@ void current_update(t_current *c)
     int axis[maxp];
     float sa[maxp], sb[maxp];
     const int np = zpic_current_passes(c, axis, sa, sb, maxp);
-    for (int k = 0; k < np; k++) {
-        ZCHECK(zk_smooth(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, axis[k], sa[k], sb[k],
-                         !c->moving_window, d->open_y, zpic_stream()));
+    for (int k = 0; k < np;) {
+        /* consecutive x passes run in one launch */
+        int run = 1;
+        while (ZPIC_SMOOTH_X_FUSE && k + run < np && run < ZK_MAX_PASSES && axis[k] == 0 && axis[k + run] == 0)
+            run++;
+        if (axis[k] == 0 && run > 1) {
+            ZCHECK(zk_smooth_x_passes(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, run, sa + k,
+                                      sb + k, !c->moving_window, zpic_stream()));
+            k += run;
+        } else {
+            ZCHECK(zk_smooth(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, axis[k], sa[k], sb[k],
+                             !c->moving_window, d->open_y, zpic_stream()));
+            k++;
+        }
         d->jcur = 1 - d->jcur;
     }
     zpic_ktimer_end(ZPIC_T_CURRENT);

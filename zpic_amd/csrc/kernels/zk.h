@@ -147,6 +147,10 @@ typedef struct zk_reduce_args {
 int zk_current_reduce(const zk_reduce_args *a, zk_stream_t s);
 int zk_smooth(const float *in, float *out, int nx, int ny, int nrow, int axis,
                      float sa, float sb, int periodic_x, int open_y, zk_stream_t s);
+/* npass (<= ZK_MAX_PASSES) consecutive x passes in one launch, bit-identical
+ * to npass zk_smooth launches along x */
+int zk_smooth_x_passes(const float *in, float *out, int nx, int ny, int nrow, int npass, const float *sa,
+                       const float *sb, int periodic_x, zk_stream_t s);
 
 /* fields */
 /* open_y: first-order Mur absorbing boundaries at the y edges */
