@@ -443,6 +443,8 @@ def main() -> int:
 
     lib.zpic_kernel_timing_reset()
     lib.zpic_kernel_timing(1)
+    st0 = (C.c_longlong * 4)()
+    lib.zpic_staged_stats(st0)
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -461,6 +463,10 @@ def main() -> int:
         tot = lib.zpic_kernel_time(k, C.byref(n))
         kt[name] = {"ms_per_launch": 1e3 * tot / max(n.value, 1), "launches": n.value}
     overflow = [lib.spec_overflow_count(s._ptr) for s in sp]
+    st1 = (C.c_longlong * 4)()
+    lib.zpic_staged_stats(st1)
+    kt["sort"].update({"staged": st1[0] - st0[0], "full_after_staged": st1[1] - st0[1],
+                       "left_without_staging_room": st1[2] - st0[2], "arrivals_without_free_slot": st1[3] - st0[3]})
     sim.close()
 
     value = npart * args.steps / elapsed

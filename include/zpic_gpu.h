@@ -76,6 +76,28 @@ void zpic_set_fixed_deposit(int on);
  * current differs from the per-species launches at FP32 rounding. */
 void zpic_set_species_fusion(int on);
 
+/* Staged re-sorts (no counterpart in em2d, whose spec_sort orders the
+ * particles by cell on the host): the push of a step that ends in a tile
+ * sort writes the particles that stayed in their tile packed to the front
+ * of the tile's range of the other buffer set and the others to a staging
+ * area, and only those are then moved to their new tiles (tile ranges keep
+ * free slots for them; arrivals that do not fit wait in an unsorted tail
+ * until the next full sort).  0 (default; every sort reads and writes all
+ * particles, ranges without free slots), 1 (environment variable
+ * ZPIC_STAGED_SORT=1) or 2 (one free slot per range, so that ranges fill
+ * up: for tests).  Results do not depend on it: the order of the particles
+ * is not part of them and the current is summed exactly.  On the 2048^2
+ * Weibel benchmark the sorting pushes (+0.4 ms each) and the spread-out
+ * ranges (+3 % per push) cost what the cheaper re-sorts save, so it is off
+ * by default. */
+void zpic_set_staged_sort(int mode);
+/* Staged re-sorts run so far in this process (a diagnostic), and with
+ * zpic_staged_stats: those, the full re-sorts that followed them (when a
+ * particle was left outside its tile), and the particles so left: leavers
+ * that found no staging room, arrivals that found no free slot. */
+long long zpic_staged_sorts(void);
+void zpic_staged_stats(long long out[4]);
+
 /* Open (absorbing) y boundaries for a simulation, before its first step.
  * No counterpart in em2d, whose y boundary is always periodic; the model
  * is the open boundary of the reference's 1D code (em1d/emf.c:379-406,

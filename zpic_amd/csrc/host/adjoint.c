@@ -270,6 +270,11 @@ static void restore_state(zpic_diff *D, int c, int iter)
         parts_copy(&d->set[d->cur], &C->p, D->np[k]);
         zpic_d2d_async(d->tile_off, C->tile_off, sizeof(int) * (size_t)(d->t.ntiles + 2));
         zpic_h2d(d->ctr, &C->ctr, sizeof(zk_spec_ctr));
+        /* packed ranges (saved after zpic_spec_sort) */
+        d->counted = 0;
+        d->n_slack = 0;
+        d->tail_upper = 0;
+        d->np_upper = C->ctr.np_total;
         d->host_valid = 0;
         s->iter = iter;
     }

@@ -157,7 +157,7 @@ static void refresh(zpic_batch *b)
         struct zpic_dev_species *d = s->dev;
         if (d->moved_since_sort) zpic_fatal("batch: species '%s' has an unsorted window shift", s->name);
         /* the batched re-sort keeps tile ranges only: no unsorted tail */
-        if (d->tail_upper > 0) zpic_spec_sort(s);
+        if (d->tail_upper > 0 || d->n_slack > 0) zpic_spec_sort(s);
         if (d->epart_cap < d->t.ntiles + ZPIC_GLOBAL_BLOCKS) zpic_fatal("batch: energy partials too small");
         zpic_current_ensure(&b->sims[b->spec_sim[k]]->current, &d->t);
     }
@@ -398,6 +398,7 @@ void zpic_batch_run(zpic_batch *b, int steps)
                 int *tmp = d->tile_off;
                 d->tile_off = d->tile_off2;
                 d->tile_off2 = tmp;
+                d->counted = 0; /* packed ranges; tile_n is not kept */
                 d->cur ^= 1;
                 d->steps_since_sort = 0;
             }

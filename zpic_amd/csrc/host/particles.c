@@ -497,6 +497,12 @@ static void species_free_device(struct zpic_dev_species *d)
     }
     zpic_dfree(d->ovf_list);
     d->ovf_list = NULL;
+    zpic_dfree(d->tile_n);
+    zpic_dfree(d->tile_n2);
+    zpic_dfree(d->sstay);
+    zpic_dfree(d->sleave);
+    d->tile_n = d->tile_n2 = d->sstay = d->sleave = NULL;
+    d->counted = 0;
     zpic_dfree(d->tile_off);
     zpic_dfree(d->tile_off2);
     zpic_dfree(d->stay);
@@ -633,31 +639,116 @@ static void grow_sets(struct zpic_dev_species *d, int need)
     d->cap = cap;
 }
 
-static void sort_now(t_species *s, int spread)
+/* Slots for n particles in tile ranges with slack (tile_cap in sort_core.h:
+ * about half as many more, the mean count per tile, room for growth, plus
+ * 32 per tile) and the staging area of the staged re-sort behind them
+ * (half the ranges). */
+static int layout_slots(long long n, int ntiles)
+{
+    const long long r = 3 * n + 32LL * ntiles;
+    long long want = r + (r >> 1) + 4096;
+    return want > ZK_MAX_PARTS ? ZK_MAX_PARTS : (int)want;
+}
+
+static long long g_staged[4]; /* staged re-sorts, full re-sorts after them, unplaced, overfull */
+long long zpic_staged_sorts(void) { return g_staged[0]; }
+void zpic_staged_stats(long long out[4])
+{
+    for (int k = 0; k < 4; k++) out[k] = g_staged[k];
+}
+
+/* Counters after a sort: slots in use, the tail, free slots, particles;
+ * returns the particles a staged re-sort could not put in their tile. */
+static int sort_readback(struct zpic_dev_species *d)
+{
+    zk_spec_ctr c;
+    zpic_d2h(&c, d->ctr, sizeof(c));
+    d->np_upper = c.np_total;
+    d->tail_upper = c.np_total - c.n_sorted;
+    d->n_slack = c.n_slack;
+    d->np_live = c.np_total - c.n_slack - c.n_dead;
+    d->counted = 1;
+    /* the early-sort probe counts global-path pushes from here */
+    d->ovf_seen = (long long)(unsigned)c.n_overflow;
+    g_staged[2] += c.n_unplaced;
+    g_staged[3] += c.n_overfull;
+    return c.n_unplaced + c.n_overfull;
+}
+
+/* Full sort into the other buffer set: `spread` for an unsorted upload
+ * (counting sort with cell interleaving), else the re-sort of tile ranges
+ * plus a tail; with `slack` the new ranges keep free slots for the staged
+ * re-sorts that follow. */
+static void sort_now_from(t_species *s, int spread, int slack, const int *before);
+static void sort_now(t_species *s, int spread, int slack)
+{
+    sort_now_from(s, spread, slack, s->dev->counted ? s->dev->tile_n : NULL);
+}
+
+/* before: the tile counts of the previous sort (NULL: unknown) */
+static void sort_now_from(t_species *s, int spread, int slack, const int *before)
 {
     struct zpic_dev_species *d = s->dev;
+    /* free slots only serve the staged re-sort (mode 2: one per range) */
+    if (slack) slack = zpic_staged_sort() == 2 ? 2 : zpic_staged_sort() ? 1 : 0;
+    if (slack) {
+        const int want = layout_slots(d->np_live, d->t.ntiles);
+        if (want > d->cap) grow_sets(d, want);
+    }
     const int other = 1 - d->cur;
     zpic_ktimer_begin(ZPIC_T_SORT);
     if (spread) {
-        /* an unsorted upload: counting sort with cell interleaving */
-        ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, &d->t, d->np_upper,
-                       spread, zpic_stream()));
+        ZCHECK(zk_sort(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->tile_off, d->cursor, d->tile_n, &d->t,
+                       d->np_upper, spread, slack, d->cap, zpic_stream()));
     } else {
         /* the particles are in tile ranges (plus a tail): keep the stayers */
         ZCHECK(zk_sort_tiles(&d->set[d->cur], &d->set[other], d->ctr, d->cnt, d->stay, d->nbr, d->tailn, d->tile_off,
-                             d->tile_off2, d->cursor, &d->t, d->tail_upper, zpic_stream()));
+                             d->counted ? d->tile_n : NULL, before, d->tile_off2, d->tile_n2, d->cursor, &d->t,
+                             d->tail_upper, slack, d->cap, zpic_stream()));
         int *tmp = d->tile_off;
         d->tile_off = d->tile_off2;
         d->tile_off2 = tmp;
+        tmp = d->tile_n;
+        d->tile_n = d->tile_n2;
+        d->tile_n2 = tmp;
     }
     zpic_ktimer_end(ZPIC_T_SORT);
     d->cur = other;
-    d->tail_upper = 0;
     d->moved_since_sort = 0;
     d->steps_since_sort = 0;
+    sort_readback(d);
 }
 
-void zpic_spec_sort(t_species *s) { sort_now(s, 0); }
+/* Whether the push of this step may sort (staged re-sort): the fixed-point
+ * push, counted ranges with free slots, and no unsorted tail. */
+static int staged_ready(const struct zpic_dev_species *d)
+{
+    return zpic_staged_sort() && zpic_fixed_deposit() && d->counted && d->n_slack > 0 && d->tail_upper == 0;
+}
+
+/* Second half of a staged re-sort (the push wrote set[1 - cur]). */
+static void resort_staged(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    g_staged[0]++;
+    zpic_ktimer_begin(ZPIC_T_SORT);
+    ZCHECK(zk_resort_staged(&d->set[1 - d->cur], d->ctr, d->tile_off, d->tile_n, d->sstay, d->sleave, d->cnt,
+                            d->tailn, d->stay, d->cursor, d->nbr, &d->t, zpic_stream()));
+    zpic_ktimer_end(ZPIC_T_SORT);
+    d->cur = 1 - d->cur;
+    /* particles left outside their tile: a full re-sort puts every one in
+     * its own tile's range */
+    if (sort_readback(d) > 0) {
+        g_staged[1]++;
+        /* sized by the growth since the sort before the staged one (the
+         * counts the staged re-sort left in sstay) */
+        sort_now_from(s, 0, 1, d->sstay);
+    }
+}
+
+/* a sort into packed ranges (no free slots), as batches and checkpoints
+ * expect */
+void zpic_spec_sort(t_species *s) { sort_now(s, 0, 0); }
 
 /* Upload the host mirror into the device buffers and sort it into tiles. */
 static void upload(t_species *s)
@@ -671,7 +762,10 @@ static void upload(t_species *s)
     if (d->track_ids && np > 0) zpic_h2d(d->set[d->cur].id, d->host_ids, sizeof(int64_t) * (size_t)np);
     d->np_upper = np;
     d->tail_upper = np;
-    sort_now(s, 1);
+    d->np_live = np;
+    /* packed ranges: the first scheduled sort makes room for staged ones
+     * (batches and checkpoints keep packed ranges and never need it) */
+    sort_now(s, 1, 0);
     d->host_dirty = 0;
 }
 
@@ -721,9 +815,16 @@ void zpic_species_ensure(t_species *s)
         d->epart_cap = d->t.ntiles + ZPIC_GLOBAL_BLOCKS;
         d->epart = zpic_dmalloc(sizeof(double) * (size_t)d->epart_cap);
         d->ready = 1;
+        d->tile_n = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->tile_n2 = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->sstay = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->sleave = zpic_dmalloc(sizeof(int) * (size_t)d->t.ntiles);
+        d->counted = 0;
+        d->n_slack = 0;
         if (d->dev_unsorted) {
             d->tail_upper = d->np_upper;
-            sort_now(s, 1);
+            d->np_live = d->np_upper;
+            sort_now(s, 1, 0);
             d->dev_unsorted = 0;
             d->host_dirty = 0;
         } else {
@@ -748,8 +849,9 @@ void spec_sync_host(t_species *s)
     if (!d || !d->ready || d->host_valid || d->host_dirty) return;
     zk_spec_ctr c;
     zpic_d2h(&c, d->ctr, sizeof(c));
-    if (c.n_dead > 0 || c.n_sorted != c.np_total) {
-        sort_now(s, 0);
+    if (c.n_dead > 0 || c.n_sorted != c.np_total || c.n_slack > 0) {
+        /* packed ranges: every slot holds a particle */
+        sort_now(s, 0, 0);
         zpic_d2h(&c, d->ctr, sizeof(c));
     }
     const int np = c.np_total;
@@ -847,6 +949,7 @@ static void inject_column(t_species *s)
         d->next_id += n;
         d->np_upper += n;
         d->tail_upper += n;
+        d->np_live += n;
     }
 }
 
@@ -861,6 +964,7 @@ static void push_args(t_species *s, t_emf *emf, t_current *cur, zk_push_args *pa
     memset(&a, 0, sizeof(a));
     a.p = d->set[d->cur];
     a.tile_off = d->tile_off;
+    a.tile_n = d->counted ? d->tile_n : NULL;
     a.ctr = d->ctr;
     a.E = zpic_emf_particle_E(emf);
     a.B = zpic_emf_particle_B(emf);
@@ -895,8 +999,7 @@ static void push_args(t_species *s, t_emf *emf, t_current *cur, zk_push_args *pa
 
 /* The rest of spec_advance after its push: iteration count, window
  * injection and energy (push_done; with fin set the energy is left to
- * the caller's zk_spec_finalize_set, which gets this species' entry),
- * then the decision whether the tiles are sorted now (sort_due). */
+ * the caller's zk_spec_finalize_set, which gets this species' entry). */
 static void push_done(t_species *s, int move, int tb, zk_finalize_set *fin)
 {
     struct zpic_dev_species *d = s->dev;
@@ -922,7 +1025,10 @@ static void push_done(t_species *s, int move, int tb, zk_finalize_set *fin)
     d->host_valid = 0;
 }
 
-static int sort_due(t_species *s)
+/* Whether the tiles are sorted after the push of this step (move: the
+ * window moves in it), decided before the push so that the push itself
+ * can sort. */
+static int sort_due(t_species *s, int move)
 {
     struct zpic_dev_species *d = s->dev;
     /* the species' schedule (em2d/particles.c:1262-1264); particles that
@@ -931,7 +1037,8 @@ static int sort_due(t_species *s)
      * order of the particles is not part of the results) */
     /* a window move shifts every particle one cell towards its tile's lower
      * x margin: after H moves the tiles are sorted again */
-    int sort = (s->n_sort > 0 && !(s->iter % s->n_sort)) || d->moved_since_sort >= (d->t.H > 1 ? d->t.H : 1);
+    const int iter = s->iter + 1, moved = d->moved_since_sort + (move ? 1 : 0);
+    int sort = (s->n_sort > 0 && !(iter % s->n_sort)) || moved >= (d->t.H > 1 ? d->t.H : 1);
     d->steps_since_sort++;
     if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
         zk_spec_ctr c;
@@ -941,6 +1048,16 @@ static int sort_due(t_species *s)
         if (novf * 1000 > (long long)ZPIC_SORT_OVF_PERMILLE * ZPIC_SORT_PROBE * (long long)c.np_total) sort = 1;
     }
     return sort;
+}
+
+/* Lets the push of this step sort the species (staged re-sort). */
+static void push_sorting(t_species *s, zk_push_args *a)
+{
+    struct zpic_dev_species *d = s->dev;
+    a->sorting = 1;
+    a->sdst = d->set[1 - d->cur];
+    a->sstay = d->sstay;
+    a->sleave = d->sleave;
 }
 
 void spec_advance(t_species *s, t_emf *emf, t_current *cur)
@@ -953,12 +1070,23 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     zk_push_args a;
     int move;
     push_args(s, emf, cur, &a, &move);
+    const int sort = sort_due(s, move);
+    const int staged = sort && staged_ready(s->dev) && a.fixed;
+    if (staged) push_sorting(s, &a);
     zpic_ktimer_begin(ZPIC_T_PUSH);
     ZCHECK(zk_push(&a, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_PUSH);
     cur->dev->fresh = 0;
+    /* a staged re-sort completes before the window's injections are
+     * appended to the (new) set */
+    if (staged) resort_staged(s);
     push_done(s, move, a.global_blocks, NULL);
-    if (sort_due(s)) sort_now(s, 0);
+    if (staged) {
+        s->dev->moved_since_sort = 0;
+        s->dev->steps_since_sort = 0;
+    } else if (sort) {
+        sort_now(s, 0, 1);
+    }
     zpic_timer_end(ZPIC_T_SPEC);
 }
 
@@ -989,21 +1117,32 @@ void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur)
             fuse = same_tiles(&m.s[k].t, &m.s[0].t) && m.s[k].fx_scale == m.s[0].fx_scale;
         if (fuse) {
             zpic_timer_begin(ZPIC_T_SPEC);
+            /* the sort decisions (whose counter reads wait for the queue)
+             * are taken before the push, which sorts where it can */
+            int due[ZK_FUSE_MAX], staged[ZK_FUSE_MAX];
+            for (int k = 0; k < n; k++) {
+                due[k] = sort_due(&spec[k], move[k]);
+                staged[k] = due[k] && staged_ready(spec[k].dev);
+                if (staged[k]) push_sorting(&spec[k], &m.s[k]);
+            }
             zpic_ktimer_begin(ZPIC_T_PUSH);
             ZCHECK(zk_push_species(&m, zpic_stream()));
             zpic_ktimer_end(ZPIC_T_PUSH);
             cur->dev->fresh = 0;
-            /* every species' energy is queued before the sort decisions,
-             * and the decisions (whose counter reads wait for the queue)
-             * are taken before any sort is queued */
-            int due[ZK_FUSE_MAX];
+            for (int k = 0; k < n; k++)
+                if (staged[k]) resort_staged(&spec[k]);
             zk_finalize_set fin;
             memset(&fin, 0, sizeof(fin));
             for (int k = 0; k < n; k++) push_done(&spec[k], move[k], m.s[k].global_blocks, &fin);
             ZCHECK(zk_spec_finalize_set(&fin, zpic_stream()));
-            for (int k = 0; k < n; k++) due[k] = sort_due(&spec[k]);
-            for (int k = 0; k < n; k++)
-                if (due[k]) sort_now(&spec[k], 0);
+            for (int k = 0; k < n; k++) {
+                if (staged[k]) {
+                    spec[k].dev->moved_since_sort = 0;
+                    spec[k].dev->steps_since_sort = 0;
+                } else if (due[k]) {
+                    sort_now(&spec[k], 0, 1);
+                }
+            }
             zpic_timer_end(ZPIC_T_SPEC);
             return;
         }

@@ -52,6 +52,18 @@ int zpic_fixed_deposit(void)
     return g_fixed_deposit;
 }
 
+/* -1: not chosen yet (environment variable ZPIC_STAGED_SORT, default off) */
+static int g_staged_sort = -1;
+void zpic_set_staged_sort(int mode) { g_staged_sort = mode == 2 ? 2 : mode ? 1 : 0; }
+int zpic_staged_sort(void)
+{
+    if (g_staged_sort < 0) {
+        const char *e = getenv("ZPIC_STAGED_SORT");
+        g_staged_sort = (e && e[0] == '2') ? 2 : (e && e[0] == '1') ? 1 : 0;
+    }
+    return g_staged_sort;
+}
+
 /* -1: not chosen yet (environment variable ZPIC_FUSE_SPECIES, default on) */
 static int g_fuse_species = -1;
 void zpic_set_species_fusion(int on) { g_fuse_species = on ? 1 : 0; }

@@ -75,6 +75,15 @@ struct zpic_dev_species {
     int cbits;            /* bits of ix in the packed cell of the device sets */
     int dev_unsorted;     /* particles made on the device by spec_new, in set[cur],
                              not yet sorted into tiles (no tiles chosen yet) */
+    /* staged re-sort (the push of a sort step sorts; only the particles
+     * that changed tile are moved afterwards) */
+    int *tile_n, *tile_n2; /* particles at the front of each tile range */
+    int counted;          /* tile_n describes the current ranges (ranges may
+                             end in free slots); 0 after code that only knows
+                             packed ranges (batches, adjoint restores) */
+    int n_slack;          /* free slots of the ranges at the last sort */
+    int np_live;          /* upper bound of the particles (not slots) */
+    int *sstay, *sleave;  /* per tile: stayers written, leavers staged */
 };
 
 struct zpic_dev_current {

@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(ZK_SORT_THREADS) count_batch(const zk_bspec *s
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     if (!sorts_now(S)) return;
     const zk_parts &p = S.set[S.pcur];
-    count_stay_body(p.cell, p.cbits, S.offs[S.ocur], S.stay, S.cnt, S.nbr, S.tailn, S.a.t, items[2 * blockIdx.x + 1],
+    count_stay_body(p.cell, p.cbits, S.offs[S.ocur], nullptr, S.stay, S.cnt, S.nbr, S.tailn, S.a.t, items[2 * blockIdx.x + 1],
                     S.a.t.ntiles);
 }
 
@@ -138,14 +138,14 @@ __global__ void __launch_bounds__(1024) scan_batch(const zk_bspec *specs)
 {
     const zk_bspec &S = specs[blockIdx.x];
     if (!sorts_now(S)) return;
-    scan_stay_body(S.stay, S.cnt, S.tailn, S.offs[1 - S.ocur], S.cursor, S.a.ctr, S.a.t.ntiles);
+    scan_stay_body(S.stay, S.cnt, S.tailn, S.offs[1 - S.ocur], nullptr, S.cursor, S.a.ctr, S.a.t.ntiles, 0, 0);
 }
 
 __global__ void __launch_bounds__(ZK_SORT_THREADS) scatter_batch(const zk_bspec *specs, const int *items)
 {
     const zk_bspec &S = specs[items[2 * blockIdx.x]];
     if (!sorts_now(S)) return;
-    scatter_stay_body(S.set[S.pcur], S.set[1 - S.pcur], S.offs[S.ocur], S.offs[1 - S.ocur], S.stay, S.nbr, S.cursor,
+    scatter_stay_body(S.set[S.pcur], S.set[1 - S.pcur], S.offs[S.ocur], nullptr, S.offs[1 - S.ocur], S.stay, S.nbr, S.cursor,
                       S.a.t, items[2 * blockIdx.x + 1], S.a.t.ntiles);
 }
 

@@ -467,6 +467,84 @@ __device__ inline void list_out(const zk_push_args &a, int i0, int i, bool out_h
     }
 }
 
+/* Stores of a push with `sorting` set (staged re-sort).  A particle that
+ * survived the step and is still in its tile goes to the front of the
+ * tile's range of the other set, one that moved to another tile to the
+ * tile's staging region behind the ranges of that set while it has room (a
+ * leaver that finds none stays with the tile; the re-sort counts it as
+ * unplaced).  sort_reserve takes the slots of a wave with one LDS atomic
+ * on a pair of counters and returns the lane's slot in sdst, or -1 for
+ * none; one set of stores then writes both kinds.  It is called by all
+ * lanes of the wave.
+ *
+ * SortSlots: the slots a sorting push writes to for one tile, the range's
+ * start and the staging region's start and size, read once per tile (the
+ * loop's stores may alias them for the compiler, whose reloads would wait
+ * for every load in flight: 2.71 -> 2.22 ms for a sorting 2048^2 push). */
+struct SortSlots {
+    int dst0, stage0, room;
+    __device__ static SortSlots of(const zk_push_args &a, int tile)
+    {
+        SortSlots r = {0, 0, 0};
+        if (a.sorting) {
+            const int lo = a.tile_off[tile] >> 1;
+            r.dst0 = a.tile_off[tile];
+            r.stage0 = a.tile_off[a.t.ntiles] + lo;
+            r.room = (a.tile_off[tile + 1] >> 1) - lo;
+        }
+        return r;
+    }
+};
+
+__device__ inline int sort_reserve(const SortSlots &ss, bool keep, bool in_tile, unsigned long long *nsl)
+{
+    const bool lv = keep && !in_tile, st = keep && in_tile;
+    const unsigned lane = __lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const unsigned long long ml = __ballot(lv), ms = __ballot(st);
+    if (!(ml | ms)) return -1;
+    /* one atomic for both counters: stayers in the low word, leavers in
+     * the high word, taken by the wave's first active lane */
+    const unsigned first = __builtin_amdgcn_readfirstlane(lane);
+    unsigned long long old = 0ull;
+    if (lane == first)
+        old = atomicAdd(nsl, (unsigned long long)__popcll(ms) | ((unsigned long long)__popcll(ml) << 32));
+    const unsigned s0 = __builtin_amdgcn_readfirstlane((unsigned)old);
+    const unsigned l0 = __builtin_amdgcn_readfirstlane((unsigned)(old >> 32));
+    int d = -1;
+    bool over = false;
+    if (st) d = ss.dst0 + (int)s0 + __popcll(ms & lt);
+    if (lv) {
+        const int r = (int)l0 + __popcll(ml & lt);
+        if (r < ss.room) d = ss.stage0 + r;
+        else over = true;
+    }
+    /* rare: leavers without staging room stay with the tile */
+    const unsigned long long mo = __ballot(over);
+    if (mo) {
+        unsigned long long o2 = 0ull;
+        if (lane == first) o2 = atomicAdd(nsl, (unsigned long long)__popcll(mo));
+        const unsigned s1 = __builtin_amdgcn_readfirstlane((unsigned)o2);
+        if (over) d = ss.dst0 + (int)s1 + __popcll(mo & lt);
+    }
+    return d;
+}
+
+__device__ inline void sort_write(const zk_push_args &a, int d, int i, const Part &c)
+{
+    if (d >= 0) {
+        store_part(a.sdst, d, c);
+        if (a.p.id) a.sdst.id[d] = a.p.id[i];
+    }
+}
+
+/* Cells [x0, x1) x [y0, y1) of a tile (the last row and column of tiles
+ * may be narrower). */
+struct TileBox {
+    int x0, x1, y0, y1;
+    __device__ bool has(int ix, int iy) const { return ix >= x0 && ix < x1 && iy >= y0 && iy < y1; }
+};
+
 /* The species a tile workgroup pushes: one (the plain and batched
  * launches), or several species of one simulation that share the tiling,
  * the fields, the current and the fixed-point scale (fused launch). */
@@ -498,12 +576,15 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
     __shared__ int ndead_blk, nlist;
     __shared__ int nls[ZK_FUSE_MAX];    /* out-of-window list lengths per species */
     __shared__ double ekin[ZK_FUSE_MAX]; /* energy partials per species */
+    __shared__ unsigned long long nsl[ZK_FUSE_MAX]; /* sorting push: stayers | leavers << 32 */
     const zk_push_args &a0 = S[0];
     const int ns = S.n();
     const zk_tiles &t = a0.t;
     if (threadIdx.x == 0) ndead_blk = nlist = 0;
+    if (threadIdx.x < ZK_FUSE_MAX) nsl[threadIdx.x] = 0ull;
 
     const int tx0 = (tile % t.ntx) * t.TX, ty0 = (tile / t.ntx) * t.TY;
+    const TileBox box = {tx0, min(tx0 + t.TX, t.nx), ty0, min(ty0 + t.TY, t.ny)};
     const int n = t.WX * t.WY;
     const int ox = tx0 - 1 - t.H, oy = ty0 - 1 - t.H; /* cell index of window node 0 */
     float *fl = sm;
@@ -549,7 +630,8 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
         zk_parts P = a.p;
         double kin = 0.0;
         int ndead = 0;
-        const int i0 = a.tile_off[tile], i1 = a.tile_off[tile + 1];
+        const int i0 = a.tile_off[tile], i1 = a.tile_n ? i0 + a.tile_n[tile] : a.tile_off[tile + 1];
+        const SortSlots ss = SortSlots::of(a, tile);
         if (PAIR == 1) {
             /* Each thread pushes particles i, i + bs, i + 2 bs, ... with the
              * loads running two particles ahead in three register sets that
@@ -575,26 +657,39 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                     int di = 0, dj = 0;
                     if (go) kin += move_one(lf, ci, cj, c.x, c.y, c.ux, c.uy, c.uz, mx, my, qvz, di, dj, k);
                     const bool cross = go && (di != 0 || dj != 0);
-                    if (go && !cross) deposit_in_cell(lj, ci, cj, c.x, c.y, mx, my, k.qnx, k.qny, qvz);
-                    dq.push(cross, ci, cj, di, dj, c.x, c.y, mx, my, qvz, lj, k);
+                    /* the particle after the step */
+                    Part nc = c;
                     if (go) {
                         float x1 = c.x + mx, y1 = c.y + my;
                         x1 -= di;
                         y1 -= dj;
-                        c.x = x1;
-                        c.y = y1;
+                        nc.x = x1;
+                        nc.y = y1;
+                        nc.ix += di;
+                        nc.iy += dj;
+                        if (boundaries(nc.ix, nc.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
+                    }
+                    int sd = -1;
+                    if (a.sorting)
+                        sd = sort_reserve(ss, go && nc.ix != ZK_DEAD, box.has(nc.ix, nc.iy), &nsl[sp]);
+                    if (go && !cross) deposit_in_cell(lj, ci, cj, c.x, c.y, mx, my, k.qnx, k.qny, qvz);
+                    dq.push(cross, ci, cj, di, dj, c.x, c.y, mx, my, qvz, lj, k);
+                    if (a.sorting) sort_write(a, sd, i, nc);
+                    else if (go) store_part(P, i, nc);
+                } else {
+                    const bool go = live && inside;
+                    if (go) {
+                        int di, dj;
+                        kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
                         c.ix += di;
                         c.iy += dj;
                         if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
-                        store_part(P, i, c);
+                        if (!a.sorting) store_part(P, i, c);
                     }
-                } else if (live && inside) {
-                    int di, dj;
-                    kin += push_one(lf, lj, c.ix - ox, c.iy - oy, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-                    c.ix += di;
-                    c.iy += dj;
-                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
-                    store_part(P, i, c);
+                    if (a.sorting) {
+                        const int sd = sort_reserve(ss, go && c.ix != ZK_DEAD, box.has(c.ix, c.iy), &nsl[sp]);
+                        sort_write(a, sd, i, c);
+                    }
                 }
             };
             /* The loop starts at the multiple of 32 at or below i0, so each
@@ -680,15 +775,25 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
         } else if (nl > 0) {
             const GlobalFields gf = {a.E, a.B, a.nrow};
             const GlobalCurrent gj = {a.Jov, a.nrow, a.jov_scale};
-            for (int q = threadIdx.x; q < nl; q += blockDim.x) {
-                const int i = a.ovf_list[i0 + q];
-                Part c = load_part(P, i);
-                int di, dj;
-                kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-                c.ix += di;
-                c.iy += dj;
-                if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
-                store_part(P, i, c);
+            /* whole waves per round: the sorting store needs every lane */
+            for (int qb = 0; qb < nl; qb += blockDim.x) {
+                const int q = qb + (int)threadIdx.x;
+                int i = 0;
+                Part c = {ZK_DEAD, 0, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                if (q < nl) {
+                    i = a.ovf_list[i0 + q];
+                    c = load_part(P, i);
+                    int di, dj;
+                    kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+                    c.ix += di;
+                    c.iy += dj;
+                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
+                    if (!a.sorting) store_part(P, i, c);
+                }
+                if (a.sorting) {
+                    const int sd = sort_reserve(ss, c.ix != ZK_DEAD, box.has(c.ix, c.iy), &nsl[sp]);
+                    sort_write(a, sd, i, c);
+                }
             }
             if (threadIdx.x == 0) atomicAdd(&a.ctr->n_overflow, nl);
         }
@@ -722,15 +827,25 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                 const GlobalFields gf = {a.E, a.B, a.nrow};
                 const GlobalCurrent gj = {a.Jov, a.nrow, a.jov_scale};
                 const int i0 = a.tile_off[tile];
-                for (int q = __lane_id(); q < nl; q += ZK_WAVE) {
-                    const int i = a.ovf_list[i0 + q];
-                    Part c = load_part(P, i);
-                    int di, dj;
-                    kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
-                    c.ix += di;
-                    c.iy += dj;
-                    if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
-                    store_part(P, i, c);
+                for (int qb = 0; qb < nl; qb += ZK_WAVE) {
+                    const int q = qb + (int)__lane_id();
+                    int i = 0;
+                    Part c = {ZK_DEAD, 0, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                    if (q < nl) {
+                        i = a.ovf_list[i0 + q];
+                        c = load_part(P, i);
+                        int di, dj;
+                        kin += push_one(gf, gj, c.ix + 1, c.iy + 1, c.x, c.y, c.ux, c.uy, c.uz, di, dj, k);
+                        c.ix += di;
+                        c.iy += dj;
+                        if (boundaries(c.ix, c.iy, t.nx, t.ny, a.moving_window, a.shift, a.open_y)) ndead++;
+                        if (!a.sorting) store_part(P, i, c);
+                    }
+                    if (a.sorting) {
+                        const SortSlots sw = SortSlots::of(a, tile);
+                        const int sd = sort_reserve(sw, c.ix != ZK_DEAD, box.has(c.ix, c.iy), &nsl[w]);
+                        sort_write(a, sd, i, c);
+                    }
                 }
                 for (int o = ZK_WAVE / 2; o > 0; o >>= 1) {
                     kin += __shfl_down(kin, o, ZK_WAVE);
@@ -742,6 +857,18 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
                 if (nl > 0) atomicAdd(&a.ctr->n_overflow, nl);
                 if (ndead) atomicAdd(&a.ctr->n_dead, ndead);
             }
+        }
+    }
+    /* sorting push: the slots taken per species */
+    __syncthreads();
+    if ((int)threadIdx.x < ns) {
+        const zk_push_args &a = S[threadIdx.x];
+        if (a.sorting) {
+            const int cap = (a.tile_off[tile + 1] >> 1) - (a.tile_off[tile] >> 1);
+            const int nl = (int)(unsigned)(nsl[threadIdx.x] >> 32);
+            a.sstay[tile] = (int)(unsigned)nsl[threadIdx.x];
+            a.sleave[tile] = min(nl, cap);
+            if (nl > cap) atomicAdd(&a.ctr->n_unplaced, nl - cap);
         }
     }
     /* store the current window of this tile */
@@ -809,7 +936,7 @@ __device__ inline void finalize_body(zk_spec_ctr *ctr, const double *epart, int 
         double tot = 0.0;
         for (int i = 0; i < (int)blockDim.x; i++) tot += part[i];
         ctr->energy = qm * tot * dx0 * dx1;
-        ctr->npush += (unsigned long long)(ctr->np_total - ctr->n_dead);
+        ctr->npush += (unsigned long long)(ctr->np_total - ctr->n_dead - ctr->n_slack);
     }
 }
 

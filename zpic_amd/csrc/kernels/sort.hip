@@ -32,9 +32,15 @@ __global__ void count_kernel(const int *__restrict__ cell, int cbits, const zk_s
     count_body(cell, cbits, ctr, cnt, t, blockIdx.x, gridDim.x);
 }
 
-__global__ void __launch_bounds__(1024) scan_kernel(int *cnt, int *off, int *cursor, zk_spec_ctr *ctr, int ntiles)
+__global__ void __launch_bounds__(1024) scan_kernel(int *cnt, int *off, int *cursor, zk_spec_ctr *ctr, int ntiles,
+                                                    int *tile_n, int slack, int cap_slots)
 {
-    scan_body(cnt, off, cursor, ctr, ntiles);
+    scan_body(cnt, off, cursor, ctr, ntiles, tile_n, slack, cap_slots);
+}
+
+__global__ void __launch_bounds__(256) mark_free_kernel(int *cell, const int *off, const int *tile_n)
+{
+    mark_free_body(cell, off, tile_n, blockIdx.x);
 }
 
 __global__ void snapshot_kernel(const zk_spec_ctr *ctr, int *n_old)
@@ -43,29 +49,51 @@ __global__ void snapshot_kernel(const zk_spec_ctr *ctr, int *n_old)
 }
 
 __global__ void scatter_indirect(zk_parts src, zk_parts dst, const int *n_old, int *cursor, const int *off,
-                                 zk_tiles t, int spread)
+                                 zk_tiles t, int spread, const int *tile_n)
 {
-    scatter_body(src, dst, n_old, cursor, off, t, spread, blockIdx.x, gridDim.x);
+    scatter_body(src, dst, n_old, cursor, off, t, spread, blockIdx.x, gridDim.x, tile_n);
 }
 
 __global__ void __launch_bounds__(ZK_SORT_THREADS)
-    count_stay_kernel(const int *__restrict__ cell, int cbits, const int *old_off, int *stay, int *move, int *nbr,
-                      int *tailn, zk_tiles t)
+    count_stay_kernel(const int *__restrict__ cell, int cbits, const int *old_off, const int *old_n, int *stay,
+                      int *move, int *nbr, int *tailn, zk_tiles t)
 {
-    count_stay_body(cell, cbits, old_off, stay, move, nbr, tailn, t, blockIdx.x, gridDim.x);
+    count_stay_body(cell, cbits, old_off, old_n, stay, move, nbr, tailn, t, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) scan_stay_kernel(const int *stay, int *move, int *tailn, int *new_off,
-                                                         int *cursor, zk_spec_ctr *ctr, int ntiles)
+                                                         int *new_n, int *cursor, zk_spec_ctr *ctr, int ntiles,
+                                                         int slack, int cap_slots, const int *old_n)
 {
-    scan_stay_body(stay, move, tailn, new_off, cursor, ctr, ntiles);
+    scan_stay_body(stay, move, tailn, new_off, new_n, cursor, ctr, ntiles, slack, cap_slots, old_n);
 }
 
 __global__ void __launch_bounds__(ZK_SORT_THREADS)
-    scatter_stay_kernel(zk_parts src, zk_parts dst, const int *old_off, const int *new_off, const int *stay,
-                        const int *nbr, int *cursor, zk_tiles t)
+    scatter_stay_kernel(zk_parts src, zk_parts dst, const int *old_off, const int *old_n, const int *new_off,
+                        const int *stay, const int *nbr, int *cursor, zk_tiles t)
 {
-    scatter_stay_body(src, dst, old_off, new_off, stay, nbr, cursor, t, blockIdx.x, gridDim.x);
+    scatter_stay_body(src, dst, old_off, old_n, new_off, stay, nbr, cursor, t, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(ZK_SORT_THREADS)
+    staged_count_kernel(const int *__restrict__ cell, int cbits, const int *off, const int *sleave, int *arr,
+                        int *far, int *nbr, zk_tiles t)
+{
+    staged_count_body(cell, cbits, off, sleave, arr, far, nbr, t, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(1024) staged_scan_kernel(const int *off, int *tile_n, int *sstay, int *arr,
+                                                           int *far, int *arr0, int *farcur, zk_spec_ctr *ctr,
+                                                           int ntiles)
+{
+    staged_scan_body(off, tile_n, sstay, arr, far, arr0, farcur, ctr, ntiles);
+}
+
+__global__ void __launch_bounds__(ZK_SORT_THREADS)
+    staged_scatter_kernel(zk_parts dst, const int *off, const int *tile_n, const int *sleave, const int *arr0,
+                          int *farcur, const int *nbr, const zk_spec_ctr *ctr, zk_tiles t)
+{
+    staged_scatter_body(dst, off, tile_n, sleave, arr0, farcur, nbr, ctr, t, blockIdx.x);
 }
 
 /* t_part AoS (28 bytes: ix, iy, x, y, ux, uy, uz) <-> SoA with the cell
@@ -124,15 +152,19 @@ unsigned grid_for(long long n, int bs, unsigned cap)
 } // namespace
 
 extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *cnt, int *off,
-                              int *cursor, const zk_tiles *t, int np_upper, int spread, zk_stream_t s)
+                              int *cursor, int *tile_n, const zk_tiles *t, int np_upper, int spread, int slack,
+                              int cap_slots, zk_stream_t s)
 {
     const unsigned g = grid_for(np_upper, 256, 8192);
     /* the slot count in use before the scan is kept in off[ntiles + 1] */
     int *n_old = off + t->ntiles + 1;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, n_old);
     hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, s, src->cell, src->cbits, ctr, cnt, *t);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles);
-    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t, spread);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles, tile_n, slack,
+                       cap_slots);
+    hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t, spread, tile_n);
+    if (t->ntiles > 0 && tile_n)
+        hipLaunchKernelGGL(mark_free_kernel, dim3(t->ntiles), dim3(256), 0, s, dst->cell, off, tile_n);
     ZK_LAUNCH_CHECK();
 }
 
@@ -142,18 +174,35 @@ extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ct
  * range with coalesced writes; particles that changed tile (and the tail)
  * are appended to their new tile. */
 extern "C" int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *move, int *stay,
-                             int *nbr, int *tailn, int *old_off, int *new_off, int *cursor, const zk_tiles *t,
-                             int tail_upper, zk_stream_t s)
+                             int *nbr, int *tailn, int *old_off, const int *old_n, const int *before,
+                             int *new_off, int *new_n, int *cursor, const zk_tiles *t, int tail_upper, int slack,
+                             int cap_slots, zk_stream_t s)
 {
     const unsigned tb = tail_upper > 0 ? grid_for(tail_upper, 1024, 4096) : 0;
     const unsigned g = (unsigned)t->ntiles + tb;
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(1), 0, s, ctr, old_off + t->ntiles + 1);
-    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, src->cell, src->cbits, old_off, stay, move,
-                       nbr, tailn, *t);
-    hipLaunchKernelGGL(scan_stay_kernel, dim3(1), dim3(1024), 0, s, stay, move, tailn, new_off, cursor, ctr,
-                       t->ntiles);
-    hipLaunchKernelGGL(scatter_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, *src, *dst, old_off, new_off, stay,
-                       nbr, cursor, *t);
+    hipLaunchKernelGGL(count_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, src->cell, src->cbits, old_off, old_n,
+                       stay, move, nbr, tailn, *t);
+    hipLaunchKernelGGL(scan_stay_kernel, dim3(1), dim3(1024), 0, s, stay, move, tailn, new_off, new_n, cursor, ctr,
+                       t->ntiles, slack, cap_slots, before);
+    hipLaunchKernelGGL(scatter_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, *src, *dst, old_off, old_n,
+                       new_off, stay, nbr, cursor, *t);
+    if (t->ntiles > 0 && new_n)
+        hipLaunchKernelGGL(mark_free_kernel, dim3(t->ntiles), dim3(256), 0, s, dst->cell, new_off, new_n);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_resort_staged(const zk_parts *dst, zk_spec_ctr *ctr, const int *tile_off, int *tile_n,
+                                int *sstay, const int *sleave, int *arr, int *far, int *arr0, int *farcur,
+                                int *nbr, const zk_tiles *t, zk_stream_t s)
+{
+    if (t->ntiles <= 0) return 0;
+    hipLaunchKernelGGL(staged_count_kernel, dim3(t->ntiles), dim3(ZK_SORT_THREADS), 0, s, dst->cell, dst->cbits,
+                       tile_off, sleave, arr, far, nbr, *t);
+    hipLaunchKernelGGL(staged_scan_kernel, dim3(1), dim3(1024), 0, s, tile_off, tile_n, sstay, arr, far, arr0, farcur,
+                       ctr, t->ntiles);
+    hipLaunchKernelGGL(staged_scatter_kernel, dim3(t->ntiles), dim3(ZK_SORT_THREADS), 0, s, *dst, tile_off, tile_n,
+                       sleave, arr0, farcur, nbr, ctr, *t);
     ZK_LAUNCH_CHECK();
 }
 

@@ -41,6 +41,12 @@ typedef struct zk_spec_ctr {
     int n_sorted;               /* end of the tile ranges                     */
     int n_dead;                 /* removed particles still occupying slots    */
     int n_overflow;             /* pushes on the global path, accumulated     */
+    int n_slack;                /* free slots at the ends of the tile ranges
+                                   (capacity left for the staged re-sort)     */
+    int n_unplaced;             /* leavers the last sorting push found no
+                                   staging room for; 0 after a full sort      */
+    int n_overfull;             /* arrivals the last staged re-sort found no
+                                   free slot for; 0 after a full sort         */
     unsigned long long npush;   /* accumulated particle pushes                */
     double energy;              /* kinetic energy of the last push            */
 } zk_spec_ctr;
@@ -67,6 +73,8 @@ typedef struct zk_tiles {
 typedef struct zk_push_args {
     zk_parts p;
     const int *tile_off;        /* [ntiles + 1] */
+    const int *tile_n;          /* [ntiles] slots in use at the front of each
+                                   range (NULL: range t ends at tile_off[t + 1]) */
     zk_spec_ctr *ctr;
     const float *E, *B;         /* fields seen by particles, buffer origin */
     int nrow;
@@ -89,6 +97,16 @@ typedef struct zk_push_args {
     float jov_scale;            /* fixed point of Jov: 2^40 / Q_J, Q_J the
                                    largest Q of the species depositing there */
     float tem, dt_dx, dt_dy, qnx, qny, q;
+    /* staged re-sort: with `sorting` set the push writes every particle
+     * that survives the step to the other buffer set `sdst`: packed at the
+     * front of its tile's range while it stays in the tile, else to its
+     * source tile's staging region behind the ranges (slots S + [off[t] / 2,
+     * off[t + 1] / 2), S = tile_off[ntiles]); sstay and sleave receive the
+     * counts per tile.  zk_resort_staged then moves the staged particles
+     * to their tiles. */
+    int sorting;
+    zk_parts sdst;
+    int *sstay, *sleave;
 } zk_push_args;
 
 /* Species of one simulation pushed by one tile launch: same tiling,
@@ -118,16 +136,37 @@ int zk_spec_finalize_set(const zk_finalize_set *f, zk_stream_t s);
 int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int nparts,
                             double qm, double dx0, double dx1, zk_stream_t s);
 
-/* tile counting sort: src -> dst; off has ntiles + 2 entries, cnt and cursor ntiles */
+/* Tile ranges.  Range t of a sorted set starts at off[t]; its first
+ * tile_n[t] slots hold the tile's particles and the rest, up to
+ * off[t + 1], are free slots marked removed (cell -1).  A sort with
+ * `slack` gives every tile room for count / 2 + 32 more particles when
+ * the set (of cap_slots slots) can also hold a staging area of half the
+ * ranges behind them; without slack, or without the room, off[t + 1] =
+ * off[t] + tile_n[t]. */
+/* tile counting sort: src -> dst; off has ntiles + 2 entries, cnt, cursor
+ * and tile_n ntiles */
 int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr,
-                   int *cnt, int *off, int *cursor, const zk_tiles *t,
-                   int np_upper, int spread, zk_stream_t s);
+                   int *cnt, int *off, int *cursor, int *tile_n, const zk_tiles *t,
+                   int np_upper, int spread, int slack, int cap_slots, zk_stream_t s);
 /* re-sort of a tile-sorted input: old_off (ntiles + 2, the last entry
- * scratch) -> new_off (ntiles + 1); move and tailn (ntiles, zero on entry
- * and exit), stay and cursor (ntiles), nbr (13 ntiles) */
+ * scratch) and old_n -> new_off (ntiles + 1) and new_n; `before` (or NULL)
+ * the counts of the previous sort, whose growth sizes the free slots; move
+ * and tailn (ntiles, zero on entry and exit), stay and cursor (ntiles),
+ * nbr (13 ntiles) */
 int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ctr, int *move, int *stay, int *nbr,
-                  int *tailn, int *old_off, int *new_off, int *cursor, const zk_tiles *t, int tail_upper,
-                  zk_stream_t s);
+                  int *tailn, int *old_off, const int *old_n, const int *before, int *new_off, int *new_n,
+                  int *cursor, const zk_tiles *t, int tail_upper, int slack, int cap_slots, zk_stream_t s);
+/* Second half of a staged re-sort, after a push with `sorting` set: the
+ * staged leavers of every tile (sleave[t] of them in the staging region of
+ * dst) go to the free slots of their new tile's range behind its sstay[t]
+ * stayers; tile_n receives the new counts and the free slots are marked
+ * removed.  Arrivals for a full range stay in the staging region, which
+ * then becomes the unsorted tail (n_overfull counts them).  arr and far
+ * (ntiles, zero on entry and exit), arr0, farcur (ntiles) and nbr (13
+ * ntiles) are scratch. */
+int zk_resort_staged(const zk_parts *dst, zk_spec_ctr *ctr, const int *tile_off, int *tile_n, int *sstay,
+                     const int *sleave, int *arr, int *far, int *arr0, int *farcur, int *nbr, const zk_tiles *t,
+                     zk_stream_t s);
 /* AoS staging <-> SoA */
 int zk_unpack(const void *aos, int n, const zk_parts *dst, int64_t id0,
                      zk_spec_ctr *ctr, int append, zk_stream_t s);
