@@ -38,17 +38,23 @@ using namespace zk;
 
 namespace {
 
-template <int MIN_WAVES, int PAIR, bool FX>
+/* WC: the window is WC x WC nodes (compile-time LDS offsets), 0 any */
+constexpr int ZK_WIN_DEFAULT = 25; /* 16 x 16 tiles, 3-cell margin */
+
+template <int MIN_WAVES, int PAIR, bool FX, int WC = 0>
 __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args a)
 {
-    push_tile_body<PAIR, FX>(OneSpecies{a}, zk_xcd_item(blockIdx.x, a.t.ntiles));
+    push_tile_body<PAIR, FX, OneSpecies, WC>(OneSpecies{a}, zk_xcd_item(blockIdx.x, a.t.ntiles));
 }
 
 /* every species of a simulation in one launch (shared staging and window) */
+template <int WC>
 __global__ void __launch_bounds__(256, ZK_PUSH_OCC) push_species_kernel(zk_push_multi m)
 {
-    push_tile_body<1, true>(SpeciesSet{m}, zk_xcd_item(blockIdx.x, m.s[0].t.ntiles));
+    push_tile_body<1, true, SpeciesSet, WC>(SpeciesSet{m}, zk_xcd_item(blockIdx.x, m.s[0].t.ntiles));
 }
+
+bool default_window(const zk_tiles &t) { return t.WX == ZK_WIN_DEFAULT && t.WY == ZK_WIN_DEFAULT; }
 
 __global__ void __launch_bounds__(256) push_global_kernel(zk_push_args a)
 {
@@ -91,7 +97,9 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
     /* variants: 0 lets the compiler keep every value in registers (about
      * 6 waves per SIMD); 8 asks for 8 waves per SIMD at the price of a few
      * register spills; 2 pushes two particles per thread and iteration */
-    const void *fn = a->fixed         ? (const void *)push_tile_kernel<ZK_PUSH_OCC, 1, true>
+    const bool dw = default_window(a->t);
+    const void *fn = a->fixed ? (dw ? (const void *)push_tile_kernel<ZK_PUSH_OCC, 1, true, ZK_WIN_DEFAULT>
+                                    : (const void *)push_tile_kernel<ZK_PUSH_OCC, 1, true>)
                      : a->waves == 8 ? (const void *)push_tile_kernel<8, 1, false>
                      : a->waves == 2 ? (const void *)push_tile_kernel<1, 2, false>
                                      : (const void *)push_tile_kernel<1, 1, false>;
@@ -101,7 +109,10 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
         if (e) return e;
     }
     if (a->t.ntiles > 0) {
-        if (a->fixed)
+        if (a->fixed && dw)
+            hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true, ZK_WIN_DEFAULT>), dim3(a->t.ntiles), dim3(256),
+                               lds, s, *a);
+        else if (a->fixed)
             hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
         else if (a->waves == 8)
             hipLaunchKernelGGL((push_tile_kernel<8, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
@@ -120,12 +131,18 @@ extern "C" int zk_push_species(const zk_push_multi *m, zk_stream_t s)
     if (m->n < 1 || m->n > ZK_FUSE_MAX || !a->fixed) return (int)hipErrorInvalidValue;
     const int n = a->t.WX * a->t.WY;
     const size_t lds = (size_t)6 * n * sizeof(float) + (size_t)3 * n * 8;
+    const bool dw = default_window(a->t);
+    const void *fn = dw ? (const void *)push_species_kernel<ZK_WIN_DEFAULT> : (const void *)push_species_kernel<0>;
     if (lds > 65536) {
-        const int e = (int)hipFuncSetAttribute((const void *)push_species_kernel,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        const int e = (int)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e) return e;
     }
-    if (a->t.ntiles > 0) hipLaunchKernelGGL(push_species_kernel, dim3(a->t.ntiles), dim3(256), lds, s, *m);
+    if (a->t.ntiles > 0) {
+        if (dw)
+            hipLaunchKernelGGL(push_species_kernel<ZK_WIN_DEFAULT>, dim3(a->t.ntiles), dim3(256), lds, s, *m);
+        else
+            hipLaunchKernelGGL(push_species_kernel<0>, dim3(a->t.ntiles), dim3(256), lds, s, *m);
+    }
     for (int k = 0; k < m->n; k++)
         if (m->s[k].global_blocks > 0)
             hipLaunchKernelGGL(push_global_kernel, dim3(m->s[k].global_blocks), dim3(256), 0, s, m->s[k]);
