@@ -668,8 +668,6 @@ static int sort_readback(struct zpic_dev_species *d)
     d->n_slack = c.n_slack;
     d->np_live = c.np_total - c.n_slack - c.n_dead;
     d->counted = 1;
-    /* the early-sort probe counts global-path pushes from here */
-    d->ovf_seen = (long long)(unsigned)c.n_overflow;
     g_staged[2] += c.n_unplaced;
     g_staged[3] += c.n_overfull;
     return c.n_unplaced + c.n_overfull;
@@ -716,7 +714,15 @@ static void sort_now_from(t_species *s, int spread, int slack, const int *before
     d->cur = other;
     d->moved_since_sort = 0;
     d->steps_since_sort = 0;
-    sort_readback(d);
+    if (slack || zpic_staged_sort()) {
+        /* the staged re-sorts need the counters; reading them waits for the
+         * queue, so the plain re-sort (packed ranges) does without */
+        sort_readback(d);
+    } else {
+        d->tail_upper = 0; /* np_upper stays an upper bound: a sort only drops slots */
+        d->counted = 1;
+        d->n_slack = 0;
+    }
 }
 
 /* Whether the push of this step may sort (staged re-sort): the fixed-point
@@ -1043,7 +1049,10 @@ static int sort_due(t_species *s, int move)
     if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
         zk_spec_ctr c;
         zpic_d2h(&c, d->ctr, sizeof(c));
-        const long long novf = (long long)(unsigned)c.n_overflow - d->ovf_seen;
+        /* global-path pushes since the later of the last probe and the last sort */
+        long long base = (long long)(unsigned)c.ovf_sorted;
+        if (base < d->ovf_seen) base = d->ovf_seen;
+        const long long novf = (long long)(unsigned)c.n_overflow - base;
         d->ovf_seen = (long long)(unsigned)c.n_overflow;
         if (novf * 1000 > (long long)ZPIC_SORT_OVF_PERMILLE * ZPIC_SORT_PROBE * (long long)c.np_total) sort = 1;
     }

@@ -136,6 +136,7 @@ __device__ inline void scan_body(int *cnt, int *off, int *cursor, zk_spec_ctr *c
         ctr->n_slack = run - part[0];
         ctr->n_unplaced = 0;
         ctr->n_overfull = 0;
+        ctr->ovf_sorted = ctr->n_overflow;
     }
 }
 
@@ -405,6 +406,7 @@ __device__ inline void scan_stay_body(const int *stay, int *move, int *tailn, in
         ctr->n_slack = total - live_s;
         ctr->n_unplaced = 0;
         ctr->n_overfull = 0;
+        ctr->ovf_sorted = ctr->n_overflow;
     }
 }
 
@@ -578,6 +580,7 @@ __device__ inline void staged_scan_body(const int *off, int *tile_n, int *sstay,
         ctr->n_dead = over_s > 0 ? (end >> 1) - over_s : 0;
         ctr->n_slack = free_s;
         ctr->n_overfull = over_s;
+        ctr->ovf_sorted = ctr->n_overflow;
     }
 }
 

@@ -47,6 +47,7 @@ typedef struct zk_spec_ctr {
                                    staging room for; 0 after a full sort      */
     int n_overfull;             /* arrivals the last staged re-sort found no
                                    free slot for; 0 after a full sort         */
+    int ovf_sorted;             /* n_overflow at the last sort               */
     unsigned long long npush;   /* accumulated particle pushes                */
     double energy;              /* kinetic energy of the last push            */
 } zk_spec_ctr;
