@@ -35,6 +35,14 @@ using namespace zk;
 #ifndef ZK_PUSH_OCC
 #define ZK_PUSH_OCC 4
 #endif
+/* waves per SIMD of the fused multi-species push: 5 fits 96 VGPRs, for
+ * which the compiler spills a few loop invariants (52 bytes per lane,
+ * stored before the loop, two 8-byte reloads per three particles); the
+ * extra wave hides more latency (2048^2 Weibel: 1.750 -> 1.727 ms per
+ * launch; 4 waves use 116 VGPRs and no scratch) */
+#ifndef ZK_SPECIES_OCC
+#define ZK_SPECIES_OCC 5
+#endif
 
 namespace {
 
@@ -49,7 +57,7 @@ __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args 
 
 /* every species of a simulation in one launch (shared staging and window) */
 template <int WC>
-__global__ void __launch_bounds__(256, ZK_PUSH_OCC) push_species_kernel(zk_push_multi m)
+__global__ void __launch_bounds__(256, ZK_SPECIES_OCC) push_species_kernel(zk_push_multi m)
 {
     push_tile_body<1, true, SpeciesSet, WC>(SpeciesSet{m}, zk_xcd_item(blockIdx.x, m.s[0].t.ntiles));
 }
