@@ -69,14 +69,14 @@ __global__ void __launch_bounds__(256) push_global_kernel(zk_push_args a)
     push_global_body(a, blockIdx.x, gridDim.x);
 }
 
-__global__ void finalize_kernel(zk_spec_ctr *ctr, const double *epart, int nparts, double qm, double dx0,
+__global__ void __launch_bounds__(1024) finalize_kernel(zk_spec_ctr *ctr, const double *epart, int nparts, double qm, double dx0,
                                 double dx1)
 {
     finalize_body(ctr, epart, nparts, qm, dx0, dx1);
 }
 
 /* one block per species */
-__global__ void finalize_set_kernel(zk_finalize_set f)
+__global__ void __launch_bounds__(1024) finalize_set_kernel(zk_finalize_set f)
 {
     const int k = blockIdx.x;
     finalize_body(f.ctr[k], f.epart[k], f.nparts[k], f.qm[k], f.dx0[k], f.dx1[k]);
@@ -160,14 +160,14 @@ extern "C" int zk_push_species(const zk_push_multi *m, zk_stream_t s)
 extern "C" int zk_spec_finalize(zk_spec_ctr *ctr, const double *epart, int nparts, double qm,
                                        double dx0, double dx1, zk_stream_t s)
 {
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, ctr, epart, nparts, qm, dx0, dx1);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, s, ctr, epart, nparts, qm, dx0, dx1);
     ZK_LAUNCH_CHECK();
 }
 
 extern "C" int zk_spec_finalize_set(const zk_finalize_set *f, zk_stream_t s)
 {
     if (f->n < 1 || f->n > ZK_FUSE_MAX) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(finalize_set_kernel, dim3(f->n), dim3(256), 0, s, *f);
+    hipLaunchKernelGGL(finalize_set_kernel, dim3(f->n), dim3(1024), 0, s, *f);
     ZK_LAUNCH_CHECK();
 }
 

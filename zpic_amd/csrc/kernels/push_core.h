@@ -942,21 +942,24 @@ __device__ inline void push_global_body(const zk_push_args &a, const int block, 
     }
 }
 
-/* Sum the energy partials in block order and count the pushes
- * (em2d/particles.c:1231, 1267). */
+/* Sum the energy partials and count the pushes (em2d/particles.c:1231,
+ * 1267).  The order of the sum is fixed (thread t adds partials t, t + B,
+ * t + 2B, ... with coalesced loads, then a tree over the B threads; B a
+ * power of two up to 1024), so every run sums alike. */
 __device__ inline void finalize_body(zk_spec_ctr *ctr, const double *epart, int nparts, double qm, double dx0,
                                      double dx1)
 {
     __shared__ double part[1024];
     double s = 0.0;
-    const int per = (nparts + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per, e = min(nparts, b + per);
-    for (int i = b; i < e; i++) s += epart[i];
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += epart[i];
     part[threadIdx.x] = s;
     __syncthreads();
+    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        double tot = 0.0;
-        for (int i = 0; i < (int)blockDim.x; i++) tot += part[i];
+        const double tot = part[0];
         ctr->energy = qm * tot * dx0 * dx1;
         ctr->npush += (unsigned long long)(ctr->np_total - ctr->n_dead - ctr->n_slack);
     }

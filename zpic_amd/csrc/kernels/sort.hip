@@ -163,7 +163,7 @@ extern "C" int zk_sort(const zk_parts *src, const zk_parts *dst, zk_spec_ctr *ct
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, off, cursor, ctr, t->ntiles, tile_n, slack,
                        cap_slots);
     hipLaunchKernelGGL(scatter_indirect, dim3(g), dim3(256), 0, s, *src, *dst, n_old, cursor, off, *t, spread, tile_n);
-    if (t->ntiles > 0 && tile_n)
+    if (t->ntiles > 0 && tile_n && slack)
         hipLaunchKernelGGL(mark_free_kernel, dim3(t->ntiles), dim3(256), 0, s, dst->cell, off, tile_n);
     ZK_LAUNCH_CHECK();
 }
@@ -187,7 +187,7 @@ extern "C" int zk_sort_tiles(const zk_parts *src, const zk_parts *dst, zk_spec_c
                        t->ntiles, slack, cap_slots, before);
     hipLaunchKernelGGL(scatter_stay_kernel, dim3(g), dim3(ZK_SORT_THREADS), 0, s, *src, *dst, old_off, old_n,
                        new_off, stay, nbr, cursor, *t);
-    if (t->ntiles > 0 && new_n)
+    if (t->ntiles > 0 && new_n && slack)
         hipLaunchKernelGGL(mark_free_kernel, dim3(t->ntiles), dim3(256), 0, s, dst->cell, new_off, new_n);
     ZK_LAUNCH_CHECK();
 }
