@@ -17,6 +17,8 @@ the results are again those of full sorts bit for bit.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -27,12 +29,14 @@ from test_gpu_parity import gpu_run
 pytestmark = pytest.mark.gpu
 
 
-def _run(gpu, cfg, mode, steps):
+def _run(gpu, cfg, mode, steps, open_y=False):
     gpu.zpic_set_staged_sort(mode)
     try:
         n0 = gpu.zpic_staged_sorts()
         r = gpu_run(gpu, cfg, ids=True)
         try:
+            if open_y:
+                gpu.zpic_sim_set_open_y(C.byref(r.sim))
             r.step(steps)
             staged = gpu.zpic_staged_sorts() - n0
             st = r.state()
@@ -60,6 +64,25 @@ def test_staged_sorts_match_full_sorts(gpu, mode):
     assert ns >= 2 * (steps // 3 - 1)
     # after either kind of sort every particle is in the range of its own
     # tile, so every tile deposits the same particles: J, E and B bit for bit
+    for f in ("E", "B", "J"):
+        assert st[f].tobytes() == full[f].tobytes(), f
+    for k in range(2):
+        assert np.array_equal(ps[k][1], pf[k][1])
+        assert ps[k][0].tobytes() == pf[k][0].tobytes(), k
+
+
+def test_staged_sorts_with_open_y_removals(gpu):
+    """Open y boundaries remove the particles that leave through y: the
+    sorting push drops them (they take no slot), and the staged runs match
+    full re-sorts bit for bit, with the same particles left."""
+    cfg = weibel(48, 40, 3, uth=0.3, n_sort=3, m_q2=-1.0)
+    steps = 30
+    full, pf, _ = _run(gpu, cfg, 0, steps, open_y=True)
+    st, ps, ns = _run(gpu, cfg, 1, steps, open_y=True)
+    print(f"open y: staged re-sorts {ns}", flush=True)
+    assert ns >= 2 * (steps // 3 - 1)
+    n0 = 48 * 40 * 9
+    assert all(len(p[1]) < n0 for p in pf)  # particles did leave
     for f in ("E", "B", "J"):
         assert st[f].tobytes() == full[f].tobytes(), f
     for k in range(2):
