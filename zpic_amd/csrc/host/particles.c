@@ -518,7 +518,7 @@ static void species_free_device(struct zpic_dev_species *d)
         if (d->pin_ev[k]) zpic_event_wait(d->pin_ev[k]);
         zpic_event_free(d->pin_ev[k]);
         zpic_pinned_free(d->pin[k]);
-        d->pin_ev[k] = d->pin[k] = NULL;
+        d->pin_ev[k] = d->pin[k] = d->pin_dev[k] = NULL;
         d->pin_cap[k] = 0;
     }
     d->pin_next = 0;
@@ -926,8 +926,11 @@ int spec_overflow_count(t_species *s)
  * ---------------------------------------------------------------------- */
 /* Inject the rightmost column after a window move (em2d/particles.c:619-640),
  * appending to the unsorted tail on the device.  The column is built in a
- * page-locked slot and copied without waiting: the host waits only when it
- * reuses a slot whose copy has not run yet. */
+ * page-locked slot that the append kernel reads directly (a copy on the
+ * stream would hold the GPU for the copy engine's start-up, about 25 us per
+ * move on the 4000 x 256 LWFA deck); the host waits only when it reuses a
+ * slot whose kernel has not run yet.  Columns of more than ZK_APPEND_MAX
+ * particles are copied and unpacked by the grid-wide kernels. */
 static void inject_column(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
@@ -941,6 +944,7 @@ static void inject_column(t_species *s)
     if (need > d->pin_cap[k]) {
         zpic_pinned_free(d->pin[k]);
         d->pin[k] = zpic_pinned_alloc(need);
+        d->pin_dev[k] = zpic_pinned_device(d->pin[k]);
         d->pin_cap[k] = need;
     }
     t_part *tmp = (t_part *)d->pin[k];
@@ -948,10 +952,15 @@ static void inject_column(t_species *s)
     inject_momenta(s, tmp, n, &d->inj);
     if (n > 0) {
         grow_sets(d, d->np_upper + n);
-        ensure_stage(d, sizeof(t_part) * (size_t)n);
-        zpic_h2d_async(d->stage, tmp, sizeof(t_part) * (size_t)n);
-        zpic_event_record(d->pin_ev[k]);
-        ZCHECK(zk_unpack(d->stage, n, &d->set[d->cur], d->next_id, d->ctr, 1, zpic_stream()));
+        if (n <= ZK_APPEND_MAX) {
+            ZCHECK(zk_unpack_append(d->pin_dev[k], n, &d->set[d->cur], d->next_id, d->ctr, zpic_stream()));
+            zpic_event_record(d->pin_ev[k]);
+        } else {
+            ensure_stage(d, sizeof(t_part) * (size_t)n);
+            zpic_h2d_async(d->stage, tmp, sizeof(t_part) * (size_t)n);
+            zpic_event_record(d->pin_ev[k]);
+            ZCHECK(zk_unpack(d->stage, n, &d->set[d->cur], d->next_id, d->ctr, 1, zpic_stream()));
+        }
         d->next_id += n;
         d->np_upper += n;
         d->tail_upper += n;

@@ -293,8 +293,11 @@ void zpic_timer_begin(int w)
     hipStream_t s = zpic_stream();
     if (!g_timer[w].created) {
         for (int k = 0; k < ZPIC_EV_PAIRS; k++) {
-            ZCHECK(hipEventCreate(&g_timer[w].ev[k][0]));
-            ZCHECK(hipEventCreate(&g_timer[w].ev[k][1]));
+            /* timing only: no system-scope fence when recorded (with one,
+             * each record wrote back and invalidated the caches and held
+             * the next kernel for about 6 us) */
+            ZCHECK(hipEventCreateWithFlags(&g_timer[w].ev[k][0], hipEventDisableSystemFence));
+            ZCHECK(hipEventCreateWithFlags(&g_timer[w].ev[k][1], hipEventDisableSystemFence));
         }
         g_timer[w].created = 1;
     }
@@ -381,11 +384,21 @@ void zpic_h2d_async(void *dst, const void *src, size_t bytes)
     if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, zpic_stream()));
 }
 
+/* Page-locked host memory, mapped and coherent: kernels read it over the
+ * bus without caching it, so a kernel sees what the host wrote before its
+ * launch (the injected columns are read this way, once each). */
 void *zpic_pinned_alloc(size_t bytes)
 {
     void *p = NULL;
-    ZCHECK(hipHostMalloc(&p, bytes > 0 ? bytes : 1, hipHostMallocDefault));
+    ZCHECK(hipHostMalloc(&p, bytes > 0 ? bytes : 1, hipHostMallocMapped | hipHostMallocCoherent));
     return p;
+}
+
+void *zpic_pinned_device(void *p)
+{
+    void *d = NULL;
+    ZCHECK(hipHostGetDevicePointer(&d, p, 0));
+    return d;
 }
 
 void zpic_pinned_free(void *p)
@@ -393,10 +406,13 @@ void zpic_pinned_free(void *p)
     if (p) ZCHECK(hipHostFree(p));
 }
 
+/* An event that tells the host when the work before it has run; it makes
+ * no device writes visible to the host (no system-scope fence), so it
+ * guards only the reuse of buffers that the device reads. */
 void *zpic_event_new(void)
 {
     hipEvent_t e;
-    ZCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ZCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
     return (void *)e;
 }
 

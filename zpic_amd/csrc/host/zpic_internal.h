@@ -56,7 +56,8 @@ struct zpic_dev_species {
     size_t stage_cap;
     void *pin[ZPIC_PIN_SLOTS];   /* page-locked staging of injected columns */
     size_t pin_cap[ZPIC_PIN_SLOTS];
-    void *pin_ev[ZPIC_PIN_SLOTS]; /* recorded after the copy out of each slot */
+    void *pin_dev[ZPIC_PIN_SLOTS]; /* the slots' device addresses */
+    void *pin_ev[ZPIC_PIN_SLOTS]; /* recorded after the last read of each slot */
     int pin_next;
     zpic_inj_scratch inj; /* host scratch of the injected columns' cell means */
     int np_upper;         /* upper bound of slots in use */

@@ -31,6 +31,7 @@ void zpic_h2d_async(void *dst, const void *src, size_t bytes); /* host buffer mu
  * on the library's stream that the host can wait for */
 void *zpic_pinned_alloc(size_t bytes);
 void zpic_pinned_free(void *p);
+void *zpic_pinned_device(void *p); /* device address of page-locked memory */
 void *zpic_event_new(void);
 void zpic_event_record(void *ev);
 void zpic_event_wait(void *ev); /* returns at once for an event never recorded */

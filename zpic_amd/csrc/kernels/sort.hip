@@ -115,6 +115,35 @@ __global__ void unpack_kernel(const int *__restrict__ aos, int n, zk_parts dst, 
     }
 }
 
+/* An injected column appended in one block: the block reads the tail's
+ * start, converts the column (read straight from page-locked host memory:
+ * no copy on the stream) and commits the new count itself, so one launch
+ * replaces a copy, the unpack and the commit. */
+__global__ void __launch_bounds__(1024) unpack_append_kernel(const int *__restrict__ aos, int n, zk_parts dst,
+                                                             int64_t id0, zk_spec_ctr *ctr)
+{
+    __shared__ int base_s;
+    if (threadIdx.x == 0) base_s = ctr->np_total;
+    __syncthreads();
+    const int base = base_s;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int *r = aos + 7 * (size_t)i;
+        int v[7];
+#pragma unroll
+        for (int c = 0; c < 7; c++) v[c] = r[c];
+        const int d = base + i;
+        dst.cell[d] = zk_cell(v[0], v[1], dst.cbits);
+        dst.x[d] = __int_as_float(v[2]);
+        dst.y[d] = __int_as_float(v[3]);
+        dst.ux[d] = __int_as_float(v[4]);
+        dst.uy[d] = __int_as_float(v[5]);
+        dst.uz[d] = __int_as_float(v[6]);
+        if (dst.id) dst.id[d] = id0 + i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ctr->np_total = base + n;
+}
+
 __global__ void unpack_commit(zk_spec_ctr *ctr, int n, int append)
 {
     if (append) {
@@ -213,6 +242,15 @@ extern "C" int zk_unpack(const void *aos, int n, const zk_parts *dst, int64_t id
         hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const int *)aos, n, *dst,
                            id0, ctr, append);
     hipLaunchKernelGGL(unpack_commit, dim3(1), dim3(1), 0, s, ctr, n, append);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_unpack_append(const void *aos, int n, const zk_parts *dst, int64_t id0, zk_spec_ctr *ctr,
+                                 zk_stream_t s)
+{
+    if (n <= 0) return 0;
+    if (n > ZK_APPEND_MAX) return 1;
+    hipLaunchKernelGGL(unpack_append_kernel, dim3(1), dim3(1024), 0, s, (const int *)aos, n, *dst, id0, ctr);
     ZK_LAUNCH_CHECK();
 }
 

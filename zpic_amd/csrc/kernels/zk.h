@@ -172,6 +172,10 @@ int zk_resort_staged(const zk_parts *dst, zk_spec_ctr *ctr, const int *tile_off,
 int zk_unpack(const void *aos, int n, const zk_parts *dst, int64_t id0,
                      zk_spec_ctr *ctr, int append, zk_stream_t s);
 int zk_pack(const zk_parts *src, int n, void *aos, zk_stream_t s);
+/* append n <= ZK_APPEND_MAX particles (t_part AoS, device-accessible, e.g.
+ * page-locked host memory) to the unsorted tail in one launch */
+#define ZK_APPEND_MAX 16384
+int zk_unpack_append(const void *aos, int n, const zk_parts *dst, int64_t id0, zk_spec_ctr *ctr, zk_stream_t s);
 
 /* current: window reduction + raw overflow -> folded J; smoothing passes */
 typedef struct zk_reduce_args {
