@@ -144,6 +144,9 @@ int spec_deposit_charge_noalloc(const t_species *spec, float *charge);
  * device: spec->part is filled by spec_sync_host (before the first step
  * as after it). */
 void zpic_set_device_init(int on);
+/* the setting in force (ZPIC_DEVICE_INIT=1 in the environment turns it on
+ * when zpic_set_device_init has not been called) */
+int zpic_device_init(void);
 /* logs of the device initialisation whose correct rounding the
  * double-double evaluation could not certify (expected 0) */
 long long zpic_device_init_ambiguous(void);

@@ -159,3 +159,52 @@ def test_batch_matches_single_gpu_runs_over_many_steps(batch_api):
         gpu.zpic_batch_delete(b)
         for g in gs:
             g.close()
+
+
+def test_device_initialised_members_match_host_initialised(batch_api):
+    """Scan members built on the device (scan.build_members(device=True))
+    hold the particles the host initialisation makes, so their batch runs
+    give the same fields and particles, bit for bit."""
+    from zpic_amd import scan
+
+    gpu = batch_api
+    seeds = [3, 8, 21]
+    params = [dict(scan.member_params(s), nx=64, ny=32 + 32 * (k % 2), ppc=2 + k) for k, s in enumerate(seeds)]
+    out = []
+    for device in (False, True):
+        prev = gpu.zpic_device_init()
+        gpu.zpic_set_device_init(1 if device else 0)
+        try:
+            ms = [scan.Member(gpu, p) for p in params]
+        finally:
+            gpu.zpic_set_device_init(prev)
+        b = scan.Batch(ms)
+        try:
+            print(f"batch of 3 members, {'device' if device else 'host'} initialisation: 25 steps", flush=True)
+            b.run(25)
+            res = []
+            for m in ms:
+                gpu.sim_sync_host(C.byref(m.sim))
+                st = {}
+                nodes = (m.sim.emf.nrow) * (m.params["ny"] + 3)
+                for f, buf in (("E", m.sim.emf.E_buf), ("B", m.sim.emf.B_buf), ("J", m.sim.current.J_buf)):
+                    st[f] = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_float)), shape=(3 * nodes,)).copy()
+                parts = []
+                for k in range(2):
+                    sp = m.species[k]
+                    raw = np.ctypeslib.as_array(C.cast(sp.part, C.POINTER(C.c_int32)), shape=(7 * sp.np,))
+                    rows = raw.reshape(sp.np, 7).copy()
+                    parts.append(rows[np.lexsort(rows.T[::-1])])
+                st["parts"] = parts
+                res.append(st)
+            out.append(res)
+        finally:
+            b.close()
+            for m in ms:
+                m.close()
+    host, dev = out
+    for h, d in zip(host, dev):
+        for f in ("E", "B", "J"):
+            assert h[f].tobytes() == d[f].tobytes(), f
+        for k in range(2):
+            assert h["parts"][k].tobytes() == d["parts"][k].tobytes(), k

@@ -85,14 +85,18 @@ class Member:
             self.sim = None
 
 
-def build_members(seeds, threads: int = 16, tiles: dict | None = None) -> list[Member]:
-    """Members for the given seeds, initialised on `threads` host threads;
-    `tiles` maps a size class (nx, ny) to the tile (tx, ty, halo) to use."""
+def build_members(seeds, threads: int = 16, tiles: dict | None = None, device: bool = False) -> list[Member]:
+    """Members for the given seeds, initialised on `threads` host threads,
+    or with `device` on the GPU (zpic_set_device_init: the same particles,
+    made member after member on this thread, which then stay on the device
+    until the batch sorts them); `tiles` maps a size class (nx, ny) to the
+    tile (tx, ty, halo) to use."""
     lib = _lib.load()
     seeds = list(seeds)
 
     def make(s):
-        lib.zpic_rand_thread_private(1)
+        if not device:
+            lib.zpic_rand_thread_private(1)
         m = Member(lib, member_params(s))
         if tiles:
             t = tiles.get(size_class(m.params))
@@ -100,6 +104,13 @@ def build_members(seeds, threads: int = 16, tiles: dict | None = None) -> list[M
                 lib.zpic_sim_set_tiling(C.addressof(m.sim), *t)
         return m
 
+    if device:
+        prev = lib.zpic_device_init()
+        lib.zpic_set_device_init(1)
+        try:
+            return [make(s) for s in seeds]
+        finally:
+            lib.zpic_set_device_init(prev)
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
         return list(ex.map(make, seeds))
 
