@@ -477,6 +477,18 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
     pthread_mutex_unlock(&g_live_lock);
 }
 
+/* The species' device counters.  A push that met a tile with more
+ * particles than the fixed-point current windows sum exactly
+ * (ZK_FX_TILE_MAX, all species of the launch) has counted it: the current
+ * of that step is not exact, so the run stops here. */
+static void ctr_read(struct zpic_dev_species *d, zk_spec_ctr *c)
+{
+    zpic_d2h(c, d->ctr, sizeof(*c));
+    if (c->n_fx_range > 0)
+        zpic_fatal("a tile holds more than %d particles, the range of the fixed-point current windows; "
+                   "use smaller tiles (zpic_set_tiling)", ZK_FX_TILE_MAX);
+}
+
 static void species_free_device(struct zpic_dev_species *d)
 {
     if (d->ready) {
@@ -533,7 +545,7 @@ static unsigned long long device_npush(struct zpic_dev_species *d)
 {
     if (!d->ready) return d->npush_retired;
     zk_spec_ctr c;
-    zpic_d2h(&c, d->ctr, sizeof(c));
+    ctr_read(d, &c);
     return d->npush_retired + c.npush;
 }
 
@@ -662,7 +674,7 @@ void zpic_staged_stats(long long out[4])
 static int sort_readback(struct zpic_dev_species *d)
 {
     zk_spec_ctr c;
-    zpic_d2h(&c, d->ctr, sizeof(c));
+    ctr_read(d, &c);
     d->np_upper = c.np_total;
     d->tail_upper = c.np_total - c.n_sorted;
     d->n_slack = c.n_slack;
@@ -854,11 +866,11 @@ void spec_sync_host(t_species *s)
     if (d && d->dev_unsorted) zpic_species_ensure(s);
     if (!d || !d->ready || d->host_valid || d->host_dirty) return;
     zk_spec_ctr c;
-    zpic_d2h(&c, d->ctr, sizeof(c));
+    ctr_read(d, &c);
     if (c.n_dead > 0 || c.n_sorted != c.np_total || c.n_slack > 0) {
         /* packed ranges: every slot holds a particle */
         sort_now(s, 0, 0);
-        zpic_d2h(&c, d->ctr, sizeof(c));
+        ctr_read(d, &c);
     }
     const int np = c.np_total;
     d->np_upper = np;
@@ -908,7 +920,7 @@ void zpic_spec_sync_energy(t_species *s)
     struct zpic_dev_species *d = s->dev;
     if (!d || !d->ready || d->host_valid) return;
     zk_spec_ctr c;
-    zpic_d2h(&c, d->ctr, sizeof(c));
+    ctr_read(d, &c);
     s->energy = c.energy;
 }
 
@@ -917,7 +929,7 @@ int spec_overflow_count(t_species *s)
     struct zpic_dev_species *d = s->dev;
     if (!d || !d->ready) return 0;
     zk_spec_ctr c;
-    zpic_d2h(&c, d->ctr, sizeof(c));
+    ctr_read(d, &c);
     return c.n_overflow;
 }
 
@@ -1057,7 +1069,7 @@ static int sort_due(t_species *s, int move)
     d->steps_since_sort++;
     if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
         zk_spec_ctr c;
-        zpic_d2h(&c, d->ctr, sizeof(c));
+        ctr_read(d, &c);
         /* global-path pushes since the later of the last probe and the last sort */
         long long base = (long long)(unsigned)c.ovf_sorted;
         if (base < d->ovf_seen) base = d->ovf_seen;

@@ -99,11 +99,15 @@ int zpic_fixed_point_exp(float q)
     return k;
 }
 
+/* Scale of the push's fixed-point current windows: 2^32 / Q.  A window
+ * node's sum must stay below 2^50 (zk_fx_read), i.e. 2^18 Q, which tiles
+ * of at most ZK_FX_TILE_MAX particles guarantee (the push counts the tiles
+ * over it); contributions are rounded to 2^-33 Q. */
 void zpic_fixed_point_scale(float q, float *scale, float *unscale)
 {
     const int k = zpic_fixed_point_exp(q);
-    *scale = ldexpf(1.0f, 40 - k);
-    *unscale = ldexpf(1.0f, k - 40);
+    *scale = ldexpf(1.0f, 32 - k);
+    *unscale = ldexpf(1.0f, k - 32);
 }
 
 void zpic_check(int status, const char *what, const char *file, int line)

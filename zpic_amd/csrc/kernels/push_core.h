@@ -71,16 +71,19 @@ struct LdsCurrentT : Win<WC> {
 };
 
 /* Fixed-point current sink: contributions are divided by Q (the power of
- * two at or above the particle charge |q|; every contribution is O(|q|)),
- * scaled by 2^40 and truncated to 64-bit integers (zk_fx), which integer
- * LDS atomics sum exactly, independent of the order of the atomics. */
+ * two at or above the particle charge |q|; a particle adds at most |q| to a
+ * node), scaled by 2^32 and rounded to integers, which integer LDS atomics
+ * sum exactly, independent of the order of the atomics.  The sums are of
+ * zk_fx_raw patterns, read back with zk_fx_read: exact while a node's sum
+ * stays below 2^18 Q, which a tile of at most ZK_FX_TILE_MAX particles
+ * guarantees (larger tiles are counted in n_fx_range). */
 template <int WC = 0>
 struct LdsCurrentFxT : Win<WC> {
     unsigned long long *j;
     float scale;
     __device__ void add(int a, int b, int c, float v) const
     {
-        atomicAdd(&j[c * this->n() + a + b * this->w()], zk_fx(v, scale));
+        atomicAdd(&j[c * this->n() + a + b * this->w()], zk_fx_raw(v, scale));
     }
 };
 
@@ -646,6 +649,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
     } else {
         lj.j = jl;
     }
+    int fx_count = 0; /* slots of the tile over all species of the launch */
     for (int sp = 0; sp < S.n(); sp++) {
         const zk_push_args &a = S[sp];
         const Consts k = {a.tem, a.dt_dx, a.dt_dy, a.qnx, a.qny, a.q};
@@ -653,6 +657,7 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
         double kin = 0.0;
         int ndead = 0;
         const int i0 = a.tile_off[tile], i1 = a.tile_n ? i0 + a.tile_n[tile] : a.tile_off[tile + 1];
+        fx_count += i1 - i0;
         const SortSlots ss = SortSlots::of(a, tile);
         if (PAIR == 1) {
             /* Each thread pushes particles i, i + bs, i + 2 bs, ... with the
@@ -893,10 +898,13 @@ __device__ __forceinline__ void push_tile_body(const Set &S, const int tile)
             if (nl > cap) atomicAdd(&a.ctr->n_unplaced, nl - cap);
         }
     }
+    /* a tile too large for the windows' exact range is counted (the host
+     * stops the run when it reads the counter) */
+    if (FX && threadIdx.x == 0 && fx_count > ZK_FX_TILE_MAX) atomicAdd(&a0.ctr->n_fx_range, 1);
     /* store the current window of this tile */
     float *dst = a0.region + (size_t)tile * 3 * n;
     for (int m = threadIdx.x; m < 3 * n; m += blockDim.x) {
-        const float v = FX ? (float)(long long)jx[m] * a0.fx_unscale : jl[m];
+        const float v = FX ? (float)zk_fx_read(jx[m]) * a0.fx_unscale : jl[m];
         dst[m] = a0.fresh ? v : dst[m] + v;
     }
 }

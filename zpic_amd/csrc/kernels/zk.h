@@ -35,6 +35,12 @@ extern "C" {
  * arrays with 32-bit byte offsets (4 * index). */
 #define ZK_MAX_PARTS (1 << 30)
 
+/* Fixed-point current windows (scale 2^32 / Q, sums read back from 51
+ * bits): every particle adds at most Q to a window node, so the sums are
+ * exact for tiles of at most this many particles (all species of a launch,
+ * free slots included). */
+#define ZK_FX_TILE_MAX ((1 << 18) - 16)
+
 /* Per-species counters kept in device memory. */
 typedef struct zk_spec_ctr {
     int np_total;               /* slots in use: tile ranges + unsorted tail */
@@ -48,6 +54,9 @@ typedef struct zk_spec_ctr {
     int n_overfull;             /* arrivals the last staged re-sort found no
                                    free slot for; 0 after a full sort         */
     int ovf_sorted;             /* n_overflow at the last sort               */
+    int n_fx_range;             /* tile pushes over more than ZK_FX_TILE_MAX
+                                   particles (the fixed-point windows' exact
+                                   range); must stay 0                        */
     unsigned long long npush;   /* accumulated particle pushes                */
     double energy;              /* kinetic energy of the last push            */
 } zk_spec_ctr;
@@ -93,8 +102,8 @@ typedef struct zk_push_args {
     int global_blocks;          /* blocks of the global (tail) pass      */
     int waves;                  /* tile-pass variant: 0, 2 or 8           */
     int fixed;                  /* 1: fixed-point (int64) current window  */
-    float fx_scale;             /* fixed point: 2^40 / Q, Q = 2^ceil(log2|q|) */
-    float fx_unscale;           /* fixed point: Q / 2^40                  */
+    float fx_scale;             /* fixed point: 2^32 / Q, Q = 2^ceil(log2|q|) */
+    float fx_unscale;           /* fixed point: Q / 2^32                  */
     float jov_scale;            /* fixed point of Jov: 2^40 / Q_J, Q_J the
                                    largest Q of the species depositing there */
     float tem, dt_dx, dt_dy, qnx, qny, q;
