@@ -270,6 +270,7 @@ static void restore_state(zpic_diff *D, int c, int iter)
         parts_copy(&d->set[d->cur], &C->p, D->np[k]);
         zpic_d2d_async(d->tile_off, C->tile_off, sizeof(int) * (size_t)(d->t.ntiles + 2));
         zpic_h2d(d->ctr, &C->ctr, sizeof(zk_spec_ctr));
+        d->probe_ready = 0;
         /* packed ranges (saved after zpic_spec_sort) */
         d->counted = 0;
         d->n_slack = 0;

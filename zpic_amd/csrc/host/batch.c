@@ -392,6 +392,7 @@ void zpic_batch_run(zpic_batch *b, int steps)
         for (int k = 0; k < b->nspecs; k++) {
             t_species *s = b->spec[k];
             s->iter++;
+            s->dev->probe_ready = 0; /* the batch's pushes change the counters */
             if (s->n_sort > 0 && s->iter % s->n_sort == 0) {
                 /* the re-sort wrote the other particle set and tile ranges */
                 struct zpic_dev_species *d = s->dev;

@@ -481,12 +481,17 @@ void spec_new(t_species *s, char name[], const float m_q, const int ppc[], const
  * particles than the fixed-point current windows sum exactly
  * (ZK_FX_TILE_MAX, all species of the launch) has counted it: the current
  * of that step is not exact, so the run stops here. */
-static void ctr_read(struct zpic_dev_species *d, zk_spec_ctr *c)
+static void ctr_check(const zk_spec_ctr *c)
 {
-    zpic_d2h(c, d->ctr, sizeof(*c));
     if (c->n_fx_range > 0)
         zpic_fatal("a tile holds more than %d particles, the range of the fixed-point current windows; "
                    "use smaller tiles (zpic_set_tiling)", ZK_FX_TILE_MAX);
+}
+
+static void ctr_read(struct zpic_dev_species *d, zk_spec_ctr *c)
+{
+    zpic_d2h(c, d->ctr, sizeof(*c));
+    ctr_check(c);
 }
 
 static void species_free_device(struct zpic_dev_species *d)
@@ -526,6 +531,11 @@ static void species_free_device(struct zpic_dev_species *d)
     zpic_dfree(d->epart);
     zpic_dfree(d->stage);
     zpic_dfree(d->ctr);
+    if (d->probe_ev) zpic_event_wait(d->probe_ev);
+    zpic_event_free(d->probe_ev);
+    zpic_pinned_free(d->probe_host);
+    d->probe_ev = d->probe_host = NULL;
+    d->probe_ready = 0;
     for (int k = 0; k < ZPIC_PIN_SLOTS; k++) {
         if (d->pin_ev[k]) zpic_event_wait(d->pin_ev[k]);
         zpic_event_free(d->pin_ev[k]);
@@ -726,6 +736,7 @@ static void sort_now_from(t_species *s, int spread, int slack, const int *before
     d->cur = other;
     d->moved_since_sort = 0;
     d->steps_since_sort = 0;
+    d->probe_ready = 0;
     if (slack || zpic_staged_sort()) {
         /* the staged re-sorts need the counters; reading them waits for the
          * queue, so the plain re-sort (packed ranges) does without */
@@ -754,6 +765,7 @@ static void resort_staged(t_species *s)
                             d->tailn, d->stay, d->cursor, d->nbr, &d->t, zpic_stream()));
     zpic_ktimer_end(ZPIC_T_SORT);
     d->cur = 1 - d->cur;
+    d->probe_ready = 0;
     /* particles left outside their tile: a full re-sort puts every one in
      * its own tile's range */
     if (sort_readback(d) > 0) {
@@ -1069,7 +1081,15 @@ static int sort_due(t_species *s, int move)
     d->steps_since_sort++;
     if (!sort && d->auto_sort && d->steps_since_sort % ZPIC_SORT_PROBE == 0) {
         zk_spec_ctr c;
-        ctr_read(d, &c);
+        if (d->probe_ready) {
+            /* the copy queued behind the last push (probe_prefetch) */
+            zpic_event_wait(d->probe_ev);
+            memcpy(&c, d->probe_host, sizeof(c));
+            ctr_check(&c);
+        } else {
+            ctr_read(d, &c);
+        }
+        d->probe_ready = 0;
         /* global-path pushes since the later of the last probe and the last sort */
         long long base = (long long)(unsigned)c.ovf_sorted;
         if (base < d->ovf_seen) base = d->ovf_seen;
@@ -1078,6 +1098,23 @@ static int sort_due(t_species *s, int move)
         if (novf * 1000 > (long long)ZPIC_SORT_OVF_PERMILLE * ZPIC_SORT_PROBE * (long long)c.np_total) sort = 1;
     }
     return sort;
+}
+
+/* At the end of a step whose successor probes the counters (sort_due),
+ * queue their copy into page-locked memory behind this step's push: the
+ * probe then reads it while the GPU runs the rest of the step (current
+ * fold, filters, field update) instead of draining the queue and leaving
+ * the GPU idle until the next push is launched.  A sort clears the copy
+ * (probe_ready), since it changes the counters. */
+static void probe_prefetch(t_species *s)
+{
+    struct zpic_dev_species *d = s->dev;
+    if (!d->auto_sort || (d->steps_since_sort + 1) % ZPIC_SORT_PROBE != 0) return;
+    if (!d->probe_host) d->probe_host = zpic_pinned_alloc(sizeof(zk_spec_ctr));
+    if (!d->probe_ev) d->probe_ev = zpic_event_new_read();
+    zpic_d2h_async(d->probe_host, d->ctr, sizeof(zk_spec_ctr));
+    zpic_event_record(d->probe_ev);
+    d->probe_ready = 1;
 }
 
 /* Lets the push of this step sort the species (staged re-sort). */
@@ -1117,6 +1154,7 @@ void spec_advance(t_species *s, t_emf *emf, t_current *cur)
     } else if (sort) {
         sort_now(s, 0, 1);
     }
+    probe_prefetch(s);
     zpic_timer_end(ZPIC_T_SPEC);
 }
 
@@ -1173,6 +1211,7 @@ void zpic_spec_advance_all(t_species *spec, int n, t_emf *emf, t_current *cur)
                     sort_now(&spec[k], 0, 1);
                 }
             }
+            for (int k = 0; k < n; k++) probe_prefetch(&spec[k]);
             zpic_timer_end(ZPIC_T_SPEC);
             return;
         }

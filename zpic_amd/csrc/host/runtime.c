@@ -187,6 +187,11 @@ void zpic_d2h(void *dst, const void *src, size_t bytes)
     ZCHECK(hipStreamSynchronize(zpic_stream()));
 }
 
+void zpic_d2h_async(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, zpic_stream()));
+}
+
 void zpic_d2d_async(void *dst, const void *src, size_t bytes)
 {
     if (bytes) ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, zpic_stream()));
@@ -417,6 +422,15 @@ void *zpic_event_new(void)
 {
     hipEvent_t e;
     ZCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+    return (void *)e;
+}
+
+/* An event after which the host reads what the work before it wrote (a
+ * copy into page-locked memory): recorded with the system-scope fence. */
+void *zpic_event_new_read(void)
+{
+    hipEvent_t e;
+    ZCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return (void *)e;
 }
 

@@ -66,6 +66,9 @@ struct zpic_dev_species {
     int steps_since_sort; /* pushes since the last tile sort */
     int auto_sort;        /* 1: also sort early when many particles left their tiles */
     long long ovf_seen;   /* overflow count at the last probe */
+    void *probe_host;     /* page-locked copy of the counters for the next probe */
+    void *probe_ev;       /* recorded after that copy */
+    int probe_ready;      /* 1: the copy was queued after the last push */
     unsigned long long npush_retired; /* pushes counted before buffers were freed */
     int64_t *host_ids;    /* ids in host-mirror order (when tracked) */
     int host_ids_cap;

@@ -26,6 +26,8 @@ void zpic_d2h(void *dst, const void *src, size_t bytes);      /* synchronous */
 void zpic_d2d_async(void *dst, const void *src, size_t bytes); /* on the stream */
 void zpic_sync(void);
 void zpic_h2d_async(void *dst, const void *src, size_t bytes); /* host buffer must outlive the copy */
+void zpic_d2h_async(void *dst, const void *src, size_t bytes); /* into page-locked memory; wait on an
+                                                                  event from zpic_event_new_read */
 
 /* page-locked host memory (copies from it run asynchronously) and events
  * on the library's stream that the host can wait for */
@@ -33,6 +35,7 @@ void *zpic_pinned_alloc(size_t bytes);
 void zpic_pinned_free(void *p);
 void *zpic_pinned_device(void *p); /* device address of page-locked memory */
 void *zpic_event_new(void);
+void *zpic_event_new_read(void); /* for data the host reads after waiting */
 void zpic_event_record(void *ev);
 void zpic_event_wait(void *ev); /* returns at once for an event never recorded */
 void zpic_event_free(void *ev);
