@@ -76,6 +76,14 @@ void zpic_set_fixed_deposit(int on);
  * current differs from the per-species launches at FP32 rounding. */
 void zpic_set_species_fusion(int on);
 
+/* Window moves written by the field update: on a step that ends in a move
+ * of the simulation window (and with no external fields) the fused Yee
+ * kernel writes the shifted grid (one column left, rightmost columns
+ * zeroed) instead of a separate pass copying the whole grid after it.
+ * 1 (default; environment variable ZPIC_FUSE_WINDOW_SHIFT=0 turns it off)
+ * or 0.  Both give the same fields bit for bit. */
+void zpic_set_window_shift_fusion(int on);
+
 /* Staged re-sorts (no counterpart in em2d, whose spec_sort orders the
  * particles by cell on the host): the push of a step that ends in a tile
  * sort writes the particles that stayed in their tile packed to the front

@@ -194,6 +194,38 @@ def test_fused_species_push_matches_per_species_launches(gpu):
         plain.close()
 
 
+@pytest.mark.parametrize("open_y", [0, 1], ids=["periodic_y", "open_y"])
+def test_window_shift_in_the_field_update_matches_the_separate_shift(gpu, open_y):
+    """zpic_set_window_shift_fusion: on a step that ends in a window move
+    the Yee kernel writes the shifted grid (em2d/emf.c:646-676) instead of
+    a copy of the whole grid after it; 40 steps of an LWFA deck (36 moves)
+    give the same fields, current and particles bit for bit."""
+    cfg = lwfa(200, 32)
+    runs = [gpu_run(gpu, cfg, ids=True), gpu_run(gpu, cfg, ids=True)]
+    try:
+        if open_y:
+            for r in runs:
+                gpu.zpic_sim_set_open_y(C.byref(r.sim))
+        for _ in range(40):
+            for on, r in zip((1, 0), runs):
+                gpu.zpic_set_window_shift_fusion(on)
+                r.step(1)
+        gpu.zpic_set_window_shift_fusion(1)
+        sa, sb = runs[0].state(), runs[1].state()
+        assert sa["emf_n_move"] == sb["emf_n_move"] >= 30
+        for f in ("E", "B", "J"):
+            assert sa[f].tobytes() == sb[f].tobytes(), f
+        for k in range(len(cfg["species"])):
+            ia, ib = gpu_ids(gpu, runs[0], k), gpu_ids(gpu, runs[1], k)
+            pa, pb = sa[f"part{k}"][np.argsort(ia)], sb[f"part{k}"][np.argsort(ib)]
+            assert np.array_equal(np.sort(ia), np.sort(ib))
+            assert pa.tobytes() == pb.tobytes(), k
+    finally:
+        gpu.zpic_set_window_shift_fusion(1)
+        for r in runs:
+            r.close()
+
+
 @pytest.mark.parametrize("case", golden_cases())
 def test_against_reference_golden_vectors(gpu, case):
     """Directly against the reference's stored states: the particle multiset

@@ -36,6 +36,7 @@ def load() -> C.CDLL:
         "zpic_sim_set_tiling": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "zpic_set_push_waves": (None, [C.c_int]),
         "zpic_set_fixed_deposit": (None, [C.c_int]),
+        "zpic_set_window_shift_fusion": (None, [C.c_int]),
         "zpic_set_staged_sort": (None, [C.c_int]),
         "zpic_staged_sorts": (C.c_longlong, []),
         "zpic_staged_stats": (None, [P(C.c_longlong)]),

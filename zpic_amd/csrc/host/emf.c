@@ -427,16 +427,25 @@ void emf_advance(t_emf *emf, const t_current *ccur)
     struct zpic_dev_emf *d = emf->dev;
     zpic_timer_begin(ZPIC_T_EMF);
     const int o = 1 - d->cur;
+    /* a window move after this step (em2d/emf.c:648-650, with the
+     * iteration count the step ends with) is written by the field update
+     * itself, unless external fields are formed from the unshifted grid in
+     * between (em2d/emf.c:703) */
+    const int move = emf->moving_window && ((emf->iter + 1) * emf->dt) > emf->dx[0] * (emf->n_move + 1);
+    const int fused = move && zpic_window_shift_fusion() && emf->ext_fld.E_type == EMF_FLD_TYPE_NONE &&
+                      emf->ext_fld.B_type == EMF_FLD_TYPE_NONE;
     zpic_ktimer_begin(ZPIC_T_FIELD);
     ZCHECK(zk_yee(d->E[d->cur], d->B[d->cur], zpic_current_J(cur), d->E[o], d->B[o], emf->nx[0], emf->nx[1],
-                  emf->nrow, emf->dt, emf->dx[0], emf->dx[1], d->open_y, zpic_stream()));
+                  emf->nrow, emf->dt, emf->dx[0], emf->dx[1], d->open_y, fused, zpic_stream()));
     ZCHECK(zk_field_guards(d->E[o], d->B[o], emf->nx[0], emf->nx[1], emf->nrow, !emf->moving_window, d->open_y,
                            zpic_stream()));
     zpic_ktimer_end(ZPIC_T_FIELD);
     d->cur = o;
     part_fields(emf);
     emf->iter += 1;
-    if (emf->moving_window && (emf->iter * emf->dt) > emf->dx[0] * (emf->n_move + 1)) {
+    if (fused) {
+        emf->n_move++;
+    } else if (move) {
         const int m = 1 - d->cur;
         ZCHECK(zk_move_window(d->E[d->cur], d->B[d->cur], d->E[m], d->B[m], emf->nx[0], emf->nx[1], emf->nrow,
                               zpic_stream()));

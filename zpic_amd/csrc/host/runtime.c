@@ -76,6 +76,18 @@ int zpic_species_fusion(void)
     return g_fuse_species;
 }
 
+/* -1: not chosen yet (environment variable ZPIC_FUSE_WINDOW_SHIFT, default on) */
+static int g_fuse_shift = -1;
+void zpic_set_window_shift_fusion(int on) { g_fuse_shift = on ? 1 : 0; }
+int zpic_window_shift_fusion(void)
+{
+    if (g_fuse_shift < 0) {
+        const char *e = getenv("ZPIC_FUSE_WINDOW_SHIFT");
+        g_fuse_shift = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_fuse_shift;
+}
+
 /* Bits of the x cell index in the packed particle cell (ix | iy << bits):
  * the least b with nx <= 2^b; the pair must fit 31 bits. */
 int zpic_cell_bits(int nx, int ny)

@@ -33,9 +33,9 @@ namespace {
 __global__ void __launch_bounds__(256) yee_kernel(const float *__restrict__ E, const float *__restrict__ B,
                                                   const float *__restrict__ J, float *__restrict__ Eo,
                                                   float *__restrict__ Bo, int nx, int ny, int nrow, YeeK k,
-                                                  int open_y)
+                                                  int open_y, int shift)
 {
-    yee_tile(E, B, J, Eo, Bo, nx, ny, nrow, k, blockIdx.x, blockIdx.y, open_y);
+    yee_tile(E, B, J, Eo, Bo, nx, ny, nrow, k, blockIdx.x, blockIdx.y, open_y, shift);
 }
 
 __global__ void guard_kernel(float *E, float *B, int nx, int ny, int nrow, int periodic_x, int open_y)
@@ -226,7 +226,7 @@ extern "C" int zk_pha(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr, i
 }
 
 extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo, int nx, int ny,
-                             int nrow, float dt, float dx, float dy, int open_y, zk_stream_t s)
+                             int nrow, float dt, float dx, float dy, int open_y, int shift, zk_stream_t s)
 {
     YeeK k;
     const float dt2 = dt / 2.0f;
@@ -237,7 +237,7 @@ extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo,
     k.dt = dt;
     k.mur_y = (dt - dy) / (dt + dy);
     dim3 grid((nx + 3 + BX - 1) / BX, (ny + 3 + BY - 1) / BY);
-    hipLaunchKernelGGL(yee_kernel, grid, dim3(256), 0, s, E, B, J, Eo, Bo, nx, ny, nrow, k, open_y);
+    hipLaunchKernelGGL(yee_kernel, grid, dim3(256), 0, s, E, B, J, Eo, Bo, nx, ny, nrow, k, open_y, shift);
     ZK_LAUNCH_CHECK();
 }
 

@@ -31,9 +31,15 @@ typedef zk_yee_k YeeK; /* (dt/2)/dx, (dt/2)/dy, dt/dx, dt/dy, dt, Mur coefficien
  * become E_in(n) + S (E_in(n+1) - E_guard(n)) with E_in the adjacent
  * interior row (0, ny - 1) and S = (dt - dy) / (dt + dy); the old values
  * are read from the input buffer E, which the kernel does not write. */
+/* shift: the step ends in a window move (em2d/emf.c:646-676): every node
+ * is written one column to the left, the leftmost guard column's values
+ * are dropped and raw columns nx - 1 .. nx + 1 are zeroed, as the
+ * reference's shift after the step leaves them (the y guards, copied after
+ * this kernel, commute with the shift). */
 __device__ inline void yee_tile(const float *__restrict__ E, const float *__restrict__ B,
                                 const float *__restrict__ J, float *__restrict__ Eo, float *__restrict__ Bo, int nx,
-                                int ny, int nrow, const YeeK &k, const int tx, const int ty, const int open_y = 0)
+                                int ny, int nrow, const YeeK &k, const int tx, const int ty, const int open_y = 0,
+                                const int shift = 0)
 {
     __shared__ float e[3][LY][LX], b[3][LY][LX], jj[3][LY][LX];
     const int bw = nx + 3, bh = ny + 3;
@@ -104,6 +110,22 @@ __device__ inline void yee_tile(const float *__restrict__ E, const float *__rest
             bx += (-k.dty2 * (e[2][v + 1][u] - e[2][v][u]));
             by += (k.dtx2 * (e[2][v][u + 1] - e[2][v][u]));
             bz += (-k.dtx2 * (e[1][v][u + 1] - e[1][v][u]) + k.dty2 * (e[0][v + 1][u] - e[0][v][u]));
+        }
+        if (shift) {
+            /* buffer column bi goes to bi - 1 (raw i - 1); raw columns from
+             * nx - 1 on are zero, the last one written by the last node */
+            if (bi == 0) continue;
+            const size_t g = 3 * ((size_t)(bi - 1) + (size_t)bj * nrow);
+            const bool zero = bi - 1 >= nx;
+            Bo[g + 0] = zero ? 0.0f : bx;
+            Bo[g + 1] = zero ? 0.0f : by;
+            Bo[g + 2] = zero ? 0.0f : bz;
+            Eo[g + 0] = zero ? 0.0f : e[0][v][u];
+            Eo[g + 1] = zero ? 0.0f : e[1][v][u];
+            Eo[g + 2] = zero ? 0.0f : e[2][v][u];
+            if (bi == bw - 1)
+                for (int c = 0; c < 3; c++) Bo[g + 3 + c] = Eo[g + 3 + c] = 0.0f;
+            continue;
         }
         const size_t g = 3 * ((size_t)bi + (size_t)bj * nrow);
         Bo[g + 0] = bx;

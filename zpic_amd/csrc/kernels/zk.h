@@ -206,9 +206,12 @@ int zk_smooth_x_passes(const float *in, float *out, int nx, int ny, int nrow, in
                        const float *sb, int periodic_x, zk_stream_t s);
 
 /* fields */
-/* open_y: first-order Mur absorbing boundaries at the y edges */
+/* open_y: first-order Mur absorbing boundaries at the y edges; shift: the
+ * output is the grid after the step and a window move (one column left,
+ * the rightmost columns zeroed), as zk_move_window would make it */
 int zk_yee(const float *E, const float *B, const float *J, float *Eo, float *Bo,
-                  int nx, int ny, int nrow, float dt, float dx, float dy, int open_y, zk_stream_t s);
+                  int nx, int ny, int nrow, float dt, float dx, float dy, int open_y, int shift,
+                  zk_stream_t s);
 int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x,
                            int open_y, zk_stream_t s);
 int zk_move_window(const float *E, const float *B, float *Eo, float *Bo, int nx,
