@@ -244,6 +244,8 @@ extern "C" int zk_yee(const float *E, const float *B, const float *J, float *Eo,
 extern "C" int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x, int open_y,
                                zk_stream_t s)
 {
+    /* a moving window (no x copies) with open y (no y copies): nothing to do */
+    if (!periodic_x && open_y) return 0;
     const int n = 3 * (nx + 3) + 3 * ny;
     hipLaunchKernelGGL(guard_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, B, nx, ny, nrow, periodic_x,
                        open_y);
