@@ -155,6 +155,16 @@ void zpic_set_device_init(int on);
 /* the setting in force (ZPIC_DEVICE_INIT=1 in the environment turns it on
  * when zpic_set_device_init has not been called) */
 int zpic_device_init(void);
+
+/* Moving-window injection on the device: with 1 (environment variable
+ * ZPIC_DEVICE_INJECT=1) each column a window move injects (uniform, step
+ * and slab profiles, em2d/particles.c:476-492) is made by one kernel,
+ * which draws the thermal momenta from a device copy of the generator
+ * state; the host takes the state back before it draws itself (rand_norm,
+ * spec_new, injection on the host), so the sequence is the reference's.
+ * 0 (default): the column is made on the host and read by the device from
+ * page-locked memory, which overlaps with the GPU's work. */
+void zpic_set_device_inject(int on);
 /* logs of the device initialisation whose correct rounding the
  * double-double evaluation could not certify (expected 0) */
 long long zpic_device_init_ambiguous(void);

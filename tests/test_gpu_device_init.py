@@ -141,3 +141,50 @@ def test_device_init_of_the_2048_benchmark_state(gpu):
     finally:
         for r in runs.values():
             r.close()
+
+
+def _inject_run(gpu, cfg, device, steps):
+    """A moving-window run with the columns injected on the device or on
+    the host; its final state, particles by id, and the next deviates of
+    the generator (drawn on the host, which takes the state back)."""
+    from test_gpu_parity import gpu_run
+    from conftest import gpu_ids
+
+    gpu.zpic_set_device_inject(1 if device else 0)
+    try:
+        r = gpu_run(gpu, cfg, ids=True)
+        try:
+            r.step(steps)
+            st = r.state()
+            ids = gpu_ids(gpu, r, 0)
+            parts = st["part0"][np.argsort(ids)]
+            nxt = [gpu.rand_norm() for _ in range(3)]
+            return st, np.sort(ids), parts, nxt, int(st["emf_n_move"])
+        finally:
+            r.close()
+    finally:
+        gpu.zpic_set_device_inject(0)
+
+
+@pytest.mark.parametrize("ppc,ny", [((4, 2), 32), ((3, 1), 31)], ids=["4x2_32", "3x1_31"])
+def test_device_injection_is_the_hosts(gpu, ppc, ny):
+    """zpic_set_device_inject: the columns of a moving window made by one
+    kernel from a device copy of the generator give the particles the host
+    injection gives (positions, momenta with the per-cell mean removed,
+    drift), bit for bit, and leave the generator where the host leaves it
+    (3 x 31 particles per column: an odd number of deviates, so the kept
+    deviate alternates between columns)."""
+    cfg = lwfa(200, ny)
+    sp = cfg["species"][0]
+    sp["ppc"] = list(ppc)
+    sp["uth"] = [0.05, 0.03, 0.02]
+    sp["ufl"] = [0.01, 0.0, -0.02]
+    print(f"moving window {ppc} ppc, {ny} rows: 40 steps, host then device injection", flush=True)
+    hs, hid, hp, hn, hm = _inject_run(gpu, cfg, False, 40)
+    ds, did, dp, dn, dm = _inject_run(gpu, cfg, True, 40)
+    assert hm == dm >= 30
+    assert np.array_equal(hid, did) and len(hid) > 0
+    assert hp.tobytes() == dp.tobytes()
+    for f in ("E", "B", "J"):
+        assert hs[f].tobytes() == ds[f].tobytes(), f
+    assert hn == dn

@@ -31,6 +31,7 @@ def load() -> C.CDLL:
         "zpic_sim_set_open_y": (None, [C.c_void_p]),
         "zpic_set_device_init": (None, [C.c_int]),
         "zpic_device_init": (C.c_int, []),
+        "zpic_set_device_inject": (None, [C.c_int]),
         "zpic_device_init_ambiguous": (C.c_longlong, []),
         "spec_deposit_charge_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
         "zpic_sim_set_tiling": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),

@@ -225,7 +225,7 @@ static void grow_sets(struct zpic_dev_species *d, int need);
 /* Logs of the polar method that the device initialisation could not
  * certify as correctly rounded (expected to stay 0). */
 static long long g_init_ambiguous = 0;
-long long zpic_device_init_ambiguous(void) { return g_init_ambiguous; }
+long long zpic_device_init_ambiguous(void) { return g_init_ambiguous + zpic_rng_device_ambiguous(); }
 
 #define ZPIC_MZ 2422800383u /* 36969 * 2^16 - 1 */
 #define ZPIC_MW 1179647999u /* 18000 * 2^16 - 1 */
@@ -955,6 +955,8 @@ int spec_overflow_count(t_species *s)
  * move on the 4000 x 256 LWFA deck); the host waits only when it reuses a
  * slot whose kernel has not run yet.  Columns of more than ZK_APPEND_MAX
  * particles are copied and unpacked by the grid-wide kernels. */
+static void inject_column_device(t_species *s, int k, zk_rng_state *rng);
+
 static void inject_column(t_species *s)
 {
     struct zpic_dev_species *d = s->dev;
@@ -964,6 +966,16 @@ static void inject_column(t_species *s)
     d->pin_next = (k + 1) % ZPIC_PIN_SLOTS;
     if (!d->pin_ev[k]) d->pin_ev[k] = zpic_event_new();
     zpic_event_wait(d->pin_ev[k]);
+    const int npc = s->ppc[0] * s->ppc[1];
+    const int dens = s->density.type;
+    if (zpic_device_inject() && (dens == UNIFORM || dens == STEP || dens == SLAB) && bound <= ZK_APPEND_MAX &&
+        sizeof(int) * (size_t)npc + sizeof(float) * 2 * (size_t)npc <= ZK_INJECT_TABLE_MAX) {
+        zk_rng_state *rng = zpic_rng_device();
+        if (rng) {
+            inject_column_device(s, k, rng);
+            return;
+        }
+    }
     const size_t need = sizeof(t_part) * (size_t)(bound > 0 ? bound : 1);
     if (need > d->pin_cap[k]) {
         zpic_pinned_free(d->pin[k]);
@@ -985,6 +997,66 @@ static void inject_column(t_species *s)
             zpic_event_record(d->pin_ev[k]);
             ZCHECK(zk_unpack(d->stage, n, &d->set[d->cur], d->next_id, d->ctr, 1, zpic_stream()));
         }
+        d->next_id += n;
+        d->np_upper += n;
+        d->tail_upper += n;
+        d->np_live += n;
+    }
+}
+
+/* The column made on the device (zk_inject_column): the host lists the
+ * lattice points of a row that the density profile keeps (as
+ * inject_positions does for each cell of the column) in page-locked slot
+ * k, and the kernel draws the momenta from the device's copy of the
+ * generator state. */
+static void inject_column_device(t_species *s, int k, zk_rng_state *rng)
+{
+    struct zpic_dev_species *d = s->dev;
+    const int npc = s->ppc[0] * s->ppc[1];
+    const size_t need = sizeof(int) * (size_t)(npc > 0 ? npc : 1) + sizeof(float) * 2 * (size_t)(npc > 0 ? npc : 1);
+    if (need > d->pin_cap[k]) {
+        zpic_pinned_free(d->pin[k]);
+        d->pin[k] = zpic_pinned_alloc(need);
+        d->pin_dev[k] = zpic_pinned_device(d->pin[k]);
+        d->pin_cap[k] = need;
+    }
+    float *lat = (float *)d->pin[k];
+    int *tab = (int *)(lat + 2 * npc);
+    const float hx = 1.0f / s->ppc[0], hy = 1.0f / s->ppc[1];
+    for (int b = 0, m = 0; b < s->ppc[1]; b++)
+        for (int a = 0; a < s->ppc[0]; a++, m++) {
+            lat[2 * m] = hx * (a + 0.5);
+            lat[2 * m + 1] = hy * (b + 0.5);
+        }
+    const int i = s->nx[0] - 1;
+    const float lo = s->density.start / s->dx[0] - s->n_move;
+    const float hi = s->density.end / s->dx[0] - s->n_move;
+    const int cut = s->density.type != UNIFORM, slab = s->density.type == SLAB;
+    int perrow = 0;
+    for (int m = 0; m < npc; m++)
+        if (!cut || (i + lat[2 * m] > lo && (!slab || i + lat[2 * m] < hi))) tab[perrow++] = m;
+    const int n = perrow * s->nx[1];
+    if (n > 0) {
+        grow_sets(d, d->np_upper + n);
+        zk_inject_args a;
+        memset(&a, 0, sizeof(a));
+        a.perrow = perrow;
+        a.rows = s->nx[1];
+        a.row0 = 0;
+        a.col = i;
+        a.tab_k = (const int *)((const float *)d->pin_dev[k] + 2 * npc);
+        a.lat = (const float *)d->pin_dev[k];
+        a.nlat = npc;
+        a.dst = d->set[d->cur];
+        a.ctr = d->ctr;
+        a.id0 = d->next_id;
+        for (int c = 0; c < 3; c++) {
+            a.uth[c] = s->uth[c];
+            a.ufl[c] = s->ufl[c];
+        }
+        a.rng = rng;
+        ZCHECK(zk_inject_column(&a, zpic_stream()));
+        zpic_event_record(d->pin_ev[k]);
         d->next_id += n;
         d->np_upper += n;
         d->tail_upper += n;

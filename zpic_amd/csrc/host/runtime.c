@@ -76,6 +76,18 @@ int zpic_species_fusion(void)
     return g_fuse_species;
 }
 
+/* -1: not chosen yet (environment variable ZPIC_DEVICE_INJECT, default off) */
+static int g_device_inject = -1;
+void zpic_set_device_inject(int on) { g_device_inject = on ? 1 : 0; }
+int zpic_device_inject(void)
+{
+    if (g_device_inject < 0) {
+        const char *e = getenv("ZPIC_DEVICE_INJECT");
+        g_device_inject = (e && e[0] == '1') ? 1 : 0;
+    }
+    return g_device_inject;
+}
+
 /* -1: not chosen yet (environment variable ZPIC_FUSE_WINDOW_SHIFT, default on) */
 static int g_fuse_shift = -1;
 void zpic_set_window_shift_fusion(int on) { g_fuse_shift = on ? 1 : 0; }

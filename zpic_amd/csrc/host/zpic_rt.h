@@ -54,6 +54,7 @@ int zpic_fixed_deposit(void);
 int zpic_device_init(void);
 int zpic_species_fusion(void);
 int zpic_window_shift_fusion(void);
+int zpic_device_inject(void);
 int zpic_staged_sort(void);
 /* fixed-point deposit constants for particle charge q (see push_core.h) */
 void zpic_fixed_point_scale(float q, float *scale, float *unscale);

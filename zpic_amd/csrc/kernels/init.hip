@@ -201,6 +201,156 @@ __global__ void apply_mean_kernel(zk_init_args a, const float *mean)
     }
 }
 
+/* One injected column in one block of 1024 threads (zk_inject_column):
+ * rounds of 4 attempts per thread, each round counted, scanned and then
+ * emitted, until the pairs are drawn; the positions, the momenta, the
+ * per-cell mean (one thread per row: a column's cells are its rows, each
+ * row's particles contiguous and summed in order, em2d/particles.c:115-129)
+ * and the generator state after the last attempt used. */
+constexpr int INJ_THREADS = 1024, INJ_PER = 4;
+
+__global__ void __launch_bounds__(1024) inject_column_kernel(zk_inject_args a)
+{
+    extern __shared__ int tab[]; /* tab_k, then the lattice (floats) */
+    __shared__ int scan[INJ_THREADS / ZK_WAVE];
+    __shared__ int base_s, last_s, total_s;
+    __shared__ double kept_s;
+    __shared__ int have_s;
+    float *lat = (float *)(tab + a.perrow);
+    const int t = threadIdx.x;
+    for (int e = t; e < a.perrow; e += INJ_THREADS) tab[e] = a.tab_k[e];
+    for (int e = t; e < 2 * a.nlat; e += INJ_THREADS) lat[e] = a.lat[e];
+    if (t == 0) {
+        base_s = a.ctr->np_total;
+        last_s = -1;
+        have_s = 0;
+        kept_s = 0.0;
+    }
+    __syncthreads();
+    const int base = base_s;
+    const long long n = (long long)a.perrow * a.rows;
+    const zk_parts &P = a.dst;
+    /* positions */
+    for (long long p = t; p < n; p += INJ_THREADS) {
+        const int j = (int)(p / a.perrow), k = tab[p - (long long)j * a.perrow];
+        const long long d = base + p;
+        P.cell[d] = zk_cell(a.col, a.row0 + j, P.cbits);
+        P.x[d] = lat[2 * k];
+        P.y[d] = lat[2 * k + 1];
+        if (P.id) P.id[d] = a.id0 + p;
+    }
+    /* thermal momenta: deviate q (after a kept one) is particle q / 3,
+     * component q mod 3 */
+    const uint32_t z0 = a.rng->z, w0 = a.rng->w;
+    const int kept0 = a.rng->have_kept;
+    const double kept_in = a.rng->kept;
+    const uint32_t z1 = 36969u * (z0 & 0xffffu) + (z0 >> 16), w1 = 18000u * (w0 & 0xffffu) + (w0 >> 16);
+    const long long D = 3 * n, pairs = (D - kept0 + 1) / 2;
+    if (kept0 && t == 0) P.ux[base] = (float)((double)a.uth[0] * kept_in);
+    int amb = 0;
+    long long pbase = 0, ab = 0;
+    /* 64 rounds hold 262144 attempts; a column needs at most 24577 pairs
+     * (ZK_APPEND_MAX particles), which take 31300 attempts on average
+     * (acceptance pi / 4): the bound only guarantees that the loop ends */
+    for (int round = 0; round < 64 && pbase < pairs; round++) {
+        const long long t0 = ab + (long long)t * INJ_PER;
+        uint32_t zz, ww;
+        if (t0 == 0) {
+            zz = z0;
+            ww = w0;
+        } else {
+            mwc_at(z1, w1, (uint64_t)(2 * t0), zz, ww);
+        }
+        const uint32_t zs = zz, ws = ww;
+        int c = 0;
+        for (int m = 0; m < INJ_PER; m++) {
+            double u, v, sq;
+            c += attempt(zz, ww, u, v, sq);
+        }
+        /* exclusive prefix of the counts over the block: within each wave
+         * by shuffles, then over the 16 wave totals */
+        const int lane = t & (ZK_WAVE - 1), wv = t / ZK_WAVE;
+        int inc = c;
+        for (int o = 1; o < ZK_WAVE; o <<= 1) {
+            const int y = __shfl_up(inc, o, ZK_WAVE);
+            if (lane >= o) inc += y;
+        }
+        if (lane == ZK_WAVE - 1) scan[wv] = inc;
+        __syncthreads();
+        if (t == 0) {
+            int run = 0;
+            for (int k = 0; k < INJ_THREADS / ZK_WAVE; k++) {
+                const int v = scan[k];
+                scan[k] = run;
+                run += v;
+            }
+            total_s = run;
+        }
+        __syncthreads();
+        long long p = pbase + scan[wv] + (inc - c);
+        zz = zs;
+        ww = ws;
+        for (int m = 0; m < INJ_PER && p < pairs; m++) {
+            double u, v, sq;
+            if (!attempt(zz, ww, u, v, sq)) continue;
+            const double fac = __builtin_sqrt(-2.0 * log_cr(sq, &amb) / sq);
+            const double dv[2] = {v * fac, u * fac};
+            for (int h = 0; h < 2; h++) {
+                const long long q = kept0 + 2 * p + h;
+                if (q < D) {
+                    const long long i = q / 3;
+                    const int cmp = (int)(q - 3 * i);
+                    float *dst = cmp == 0 ? P.ux : (cmp == 1 ? P.uy : P.uz);
+                    dst[base + i] = (float)((double)a.uth[cmp] * dv[h]);
+                } else {
+                    kept_s = dv[h];
+                    have_s = 1;
+                }
+            }
+            if (p == pairs - 1) last_s = (int)(t0 + m);
+            p++;
+        }
+        pbase += total_s;
+        ab += (long long)INJ_THREADS * INJ_PER;
+        __syncthreads();
+    }
+    if (amb) atomicAdd(&a.rng->ambiguous, amb);
+    /* per-cell mean (the column's cells are its rows), then the drift */
+    for (int j = t; j < a.rows; j += INJ_THREADS) {
+        const long long p0 = base + (long long)j * a.perrow;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (int m = 0; m < a.perrow; m++) {
+            sx += P.ux[p0 + m];
+            sy += P.uy[p0 + m];
+            sz += P.uz[p0 + m];
+        }
+        if (a.perrow > 0) {
+            const float wgt = 1.0f / a.perrow;
+            sx *= wgt;
+            sy *= wgt;
+            sz *= wgt;
+        }
+        for (int m = 0; m < a.perrow; m++) {
+            P.ux[p0 + m] += a.ufl[0] - sx;
+            P.uy[p0 + m] += a.ufl[1] - sy;
+            P.uz[p0 + m] += a.ufl[2] - sz;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        /* the generator after the calls of the last attempt used */
+        if (pairs > 0) {
+            uint32_t zz, ww;
+            mwc_at(z1, w1, (uint64_t)(2 * (long long)last_s + 2), zz, ww);
+            a.rng->z = zz;
+            a.rng->w = ww;
+            a.rng->have_kept = have_s;
+            a.rng->kept = have_s ? kept_s : 0.0;
+        }
+        a.ctr->np_total = base + (int)n;
+    }
+}
+
 unsigned blocks_for(long long n, unsigned cap)
 {
     long long b = (n + 255) / 256;
@@ -237,5 +387,14 @@ extern "C" int zk_init_mean(const zk_init_args *a, float *mean, int nkeys, zk_st
 {
     if (nkeys > 0) hipLaunchKernelGGL(cell_mean_kernel, dim3((nkeys + 255) / 256), dim3(256), 0, s, *a, mean, nkeys);
     if (a->np > 0) hipLaunchKernelGGL(apply_mean_kernel, dim3(blocks_for(a->np, 16384)), dim3(256), 0, s, *a, mean);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_inject_column(const zk_inject_args *a, zk_stream_t s)
+{
+    const long long n = (long long)a->perrow * a->rows;
+    const size_t lds = sizeof(int) * (size_t)a->perrow + sizeof(float) * 2 * (size_t)a->nlat;
+    if (n <= 0 || n > ZK_APPEND_MAX || lds > ZK_INJECT_TABLE_MAX) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(inject_column_kernel, dim3(1), dim3(INJ_THREADS), lds, s, *a);
     ZK_LAUNCH_CHECK();
 }

@@ -255,6 +255,36 @@ typedef struct zk_init_args {
     double kept_in;
     long long pairs;                    /* pairs of deviates drawn */
 } zk_init_args;
+/* The injection generator's state on the device (the host keeps its own
+ * copy in random.c and hands the state over in either direction). */
+typedef struct zk_rng_state {
+    uint32_t w, z;                      /* multiply-with-carry halves */
+    double kept;                        /* deviate kept by the polar method */
+    int have_kept;
+    int ambiguous;                      /* logs not certified as correctly rounded, accumulated */
+} zk_rng_state;
+
+/* One injected column of a moving window (em2d/particles.c:476-492 with
+ * spec_set_x / spec_set_u, :85-354), made on the device and appended to the
+ * unsorted tail: rows x perrow particles, the row's lattice points tab_k,
+ * thermal momenta from the generator state *rng (continued there), the
+ * per-cell mean removed, the drift added. */
+typedef struct zk_inject_args {
+    int perrow, rows, row0, col;
+    const int *tab_k;                   /* [perrow] lattice point of each particle of a row */
+    const float *lat;                   /* [2 nlat] lattice points */
+    int nlat;
+    zk_parts dst;
+    zk_spec_ctr *ctr;
+    int64_t id0;
+    float uth[3], ufl[3];
+    zk_rng_state *rng;
+} zk_inject_args;
+/* at most ZK_APPEND_MAX particles and ZK_INJECT_TABLE_MAX bytes of
+ * tab_k and lattice */
+#define ZK_INJECT_TABLE_MAX (48 * 1024)
+int zk_inject_column(const zk_inject_args *a, zk_stream_t s);
+
 typedef struct zk_init_out {
     long long last_attempt;             /* attempt that gave the last pair */
     double kept;                        /* deviate left over (have_kept) */
