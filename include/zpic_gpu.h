@@ -150,7 +150,8 @@ int spec_deposit_charge_noalloc(const t_species *spec, float *charge);
  * about 0.06 % of the logs), so a momentum can in rare cases differ from
  * the host initialisation's in its last bit.  The particles stay on the
  * device: spec->part is filled by spec_sync_host (before the first step
- * as after it). */
+ * as after it).  emf_add_laser then also makes its pulse on the device
+ * (em2d/emf.c:158-350, the same expressions). */
 void zpic_set_device_init(int on);
 /* the setting in force (ZPIC_DEVICE_INIT=1 in the environment turns it on
  * when zpic_set_device_init has not been called) */

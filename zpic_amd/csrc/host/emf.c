@@ -185,14 +185,41 @@ void emf_add_laser(t_emf *const emf, t_emf_laser *L)
         fprintf(stderr, "Invalid laser FALL, must be > 0, aborting.\n");
         exit(-1);
     }
+    const float amp = L->omega0 * L->a0;
+    const float cp = cos(L->polarization);
+    const float sp = sin(L->polarization);
+    if (zpic_device_init() && (L->type == PLANE || L->type == GAUSSIAN)) {
+        /* on the device (zpic_set_device_init): the same expressions, the
+         * guards refreshed as host_update_gc does */
+        zpic_emf_ensure(emf);
+        struct zpic_dev_emf *d = emf->dev;
+        zk_laser_k k;
+        k.start = L->start;
+        k.rise = L->rise;
+        k.flat = L->flat;
+        k.fall = L->fall;
+        k.focus = L->focus;
+        k.omega0 = L->omega0;
+        k.W0 = L->W0;
+        k.axis = L->axis;
+        k.amp = amp;
+        k.cp = cp;
+        k.sp = sp;
+        k.hx = emf->dx[0];
+        k.hy = emf->dx[1];
+        k.type = L->type == PLANE ? 0 : 1;
+        ZCHECK(zk_laser(d->E[d->cur], d->B[d->cur], emf->nx[0], emf->nx[1], emf->nrow, &k,
+                        (double)(emf->dx[0] / emf->dx[1]), zpic_stream()));
+        ZCHECK(zk_field_guards(d->E[d->cur], d->B[d->cur], emf->nx[0], emf->nx[1], emf->nrow, !emf->moving_window,
+                               0, zpic_stream()));
+        d->host_valid = 0;
+        return;
+    }
     host_edit_begin(emf);
     float3 *E = emf->E, *B = emf->B;
     const int nrow = emf->nrow;
     const float hx = emf->dx[0], hy = emf->dx[1];
     const float axis = L->axis;
-    const float amp = L->omega0 * L->a0;
-    const float cp = cos(L->polarization);
-    const float sp = sin(L->polarization);
     if (L->type == PLANE) {
         const float k = L->omega0;
         for (int i = 0; i < emf->nx[0]; i++) {

@@ -60,6 +60,86 @@ __global__ void move_window_kernel(const float *E, const float *B, float *Eo, fl
     }
 }
 
+/* ---- laser pulse (em2d/emf.c:158-350), on the device ----
+ * The reference's expressions: float operands, the C library's double
+ * functions (sin, cos, exp, atan2, sqrt) with the results rounded to float
+ * where the reference assigns them; the device's double functions agree
+ * with the host's to within an ulp of double, so a float result can differ
+ * only where the double value lies that close to a float rounding
+ * boundary. */
+__device__ inline float laser_envelope(const zk_laser_k &L, const float z)
+{
+    if (z > L.start) return 0.0f;
+    if (z > L.start - L.rise) {
+        const float u = z - L.start;
+        const float e = (float)sin(1.5707963267948966 * u / L.rise);
+        return e * e;
+    }
+    if (z > L.start - (L.rise + L.flat)) return 1.0f;
+    if (z > L.start - (L.rise + L.flat + L.fall)) {
+        const float u = z - (L.start - L.rise - L.flat - L.fall);
+        const float e = (float)sin(1.5707963267948966 * u / L.fall);
+        return e * e;
+    }
+    return 0.0f;
+}
+
+__device__ inline float laser_phase(const zk_laser_k &L, const float zpos, const float r)
+{
+    const float z = zpos - L.focus;
+    const float zr = L.omega0 * (L.W0 * L.W0) / 2;
+    const float r2 = r * r;
+    const float curv = r2 * z / (zr * zr + z * z);
+    const float w2 = (zr * zr) / (zr * zr + z * z);
+    const float gouy = (float)atan2((double)z, (double)zr);
+    return (float)(sqrt(sqrt((double)w2)) * exp((double)(-r2 * w2 / (L.W0 * L.W0))) *
+                   cos((double)(L.omega0 * (z + curv) - gouy)));
+}
+
+/* one thread per interior node; E, B from the buffer origin */
+__global__ void laser_kernel(float *E, float *B, int nx, int ny, int nrow, zk_laser_k L)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nx * ny) return;
+    const int i = node % nx, j = node / nx;
+    const size_t g = 3 * ((size_t)(i + 1) + (size_t)(j + 1) * nrow);
+    const float z = i * L.hx;
+    const float zh = z + L.hx / 2;
+    const float env = L.amp * laser_envelope(L, z);
+    const float envh = L.amp * laser_envelope(L, zh);
+    if (L.type == 0) { /* plane: the products and the sums in double, as in the reference */
+        const double c = cos((double)(L.omega0 * z)), ch = cos((double)(L.omega0 * zh));
+        E[g + 1] = (float)((double)E[g + 1] + (double)env * c * (double)L.cp);
+        E[g + 2] = (float)((double)E[g + 2] + (double)env * c * (double)L.sp);
+        B[g + 1] = (float)((double)B[g + 1] + (double)(-envh) * ch * (double)L.sp);
+        B[g + 2] = (float)((double)B[g + 2] + (double)envh * ch * (double)L.cp);
+    } else {
+        const float r = j * L.hy - L.axis;
+        const float rh = r + L.hy / 2;
+        E[g + 1] += +env * laser_phase(L, z, rh) * L.cp;
+        E[g + 2] += +env * laser_phase(L, z, r) * L.sp;
+        B[g + 1] += -envh * laser_phase(L, zh, r) * L.sp;
+        B[g + 2] += +envh * laser_phase(L, zh, rh) * L.cp;
+    }
+}
+
+/* E_x, B_x from div E = div B = 0, integrated in double from the right
+ * edge of each row (em2d/emf.c:213-231): one thread per row */
+__global__ void laser_div_kernel(float *E, float *B, int nx, int ny, int nrow, double ratio)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ny) return;
+#define AT(a, b) (3 * ((size_t)((a) + 1) + (size_t)((b) + 1) * nrow))
+    double ex = 0.0, bx = 0.0;
+    for (int i = nx - 1; i >= 0; i--) {
+        ex += ratio * (E[AT(i + 1, j) + 1] - E[AT(i + 1, j - 1) + 1]);
+        E[AT(i, j)] = (float)ex;
+        bx += ratio * (B[AT(i, j + 1) + 1] - B[AT(i, j) + 1]);
+        B[AT(i, j)] = (float)bx;
+    }
+#undef AT
+}
+
 __global__ void add_ext_kernel(const float *F, const float *ext, float ux, float uy, float uz, float *out,
                                size_t nodes)
 {
@@ -284,5 +364,14 @@ extern "C" int zk_charge(const zk_parts *p, int np_upper, const zk_spec_ctr *ctr
     if (b < 1) b = 1;
     if (b > 4096) b = 4096;
     hipLaunchKernelGGL(charge_kernel, dim3((unsigned)b), dim3(256), 0, s, *p, ctr, q, nx, rho);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_laser(float *E, float *B, int nx, int ny, int nrow, const zk_laser_k *L, double ratio, zk_stream_t s)
+{
+    const int nodes = nx * ny;
+    if (nodes <= 0) return 0;
+    hipLaunchKernelGGL(laser_kernel, dim3((nodes + 255) / 256), dim3(256), 0, s, E, B, nx, ny, nrow, *L);
+    if (L->type != 0) hipLaunchKernelGGL(laser_div_kernel, dim3((ny + 63) / 64), dim3(64), 0, s, E, B, nx, ny, nrow, ratio);
     ZK_LAUNCH_CHECK();
 }

@@ -216,6 +216,14 @@ int zk_field_guards(float *E, float *B, int nx, int ny, int nrow, int periodic_x
                            int open_y, zk_stream_t s);
 int zk_move_window(const float *E, const float *B, float *Eo, float *Bo, int nx,
                           int ny, int nrow, zk_stream_t s);
+/* laser pulse added to E, B (buffer origins; em2d/emf.c:242-350): type 0
+ * plane, 1 gaussian (then the longitudinal components from div E = div B
+ * = 0, ratio = dx / dy); guards are left to zk_field_guards */
+typedef struct zk_laser_k {
+    float start, rise, flat, fall, focus, omega0, W0, axis, amp, cp, sp, hx, hy;
+    int type;
+} zk_laser_k;
+int zk_laser(float *E, float *B, int nx, int ny, int nrow, const zk_laser_k *L, double ratio, zk_stream_t s);
 /* out = F + ext[node] when ext is given, else F + (ux,uy,uz), over every node */
 int zk_add_ext(const float *F, const float *ext, float ux, float uy, float uz,
                       float *out, size_t nodes, zk_stream_t s);
