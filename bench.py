@@ -305,9 +305,9 @@ def lwfa_workload(args) -> dict:
     e = em2d.Species("electrons", -1.0, [4, 2], density=em2d.Density(type="step", start=20.0))
     sim = em2d.Simulation([nx, ny], [nx * dx, ny * dy], dt, species=[e], mov_window=True,
                           smooth=em2d.Smooth(xtype="compensated", xlevel=4), open_y=True)
-    lib.zpic_set_device_init(0)
     sim.add_laser(em2d.Laser(type="gaussian", start=17.0, fwhm=2.0, a0=2.0, omega0=10.0, W0=4.0, focus=20.0,
                              axis=25.6, polarization=0.0))
+    lib.zpic_set_device_init(0)
     npart = int(e._s.np)
     warm, steps = 10, 1990
     for _ in range(warm):
