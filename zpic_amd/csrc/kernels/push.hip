@@ -49,9 +49,18 @@ namespace {
 /* WC: the window is WC x WC nodes (compile-time LDS offsets), 0 any */
 constexpr int ZK_WIN_DEFAULT = 25; /* 16 x 16 tiles, 3-cell margin */
 
+/* Blocks past the tiles push the unsorted tail (the global pass) in the
+ * same launch: the two touch disjoint particles, deposit into disjoint
+ * windows or with exact atomics into the overflow grid, and write their own
+ * energy partials, so they may run side by side (one launch and its gap
+ * less per step while a moving window's injected columns wait for a sort). */
 template <int MIN_WAVES, int PAIR, bool FX, int WC = 0>
 __global__ void __launch_bounds__(256, MIN_WAVES) push_tile_kernel(zk_push_args a)
 {
+    if ((int)blockIdx.x >= a.t.ntiles) {
+        push_global_body(a, (int)blockIdx.x - a.t.ntiles, a.global_blocks);
+        return;
+    }
     push_tile_body<PAIR, FX, OneSpecies, WC>(OneSpecies{a}, zk_xcd_item(blockIdx.x, a.t.ntiles));
 }
 
@@ -117,19 +126,21 @@ extern "C" int zk_push(const zk_push_args *a, zk_stream_t s)
         if (e) return e;
     }
     if (a->t.ntiles > 0) {
+        /* the tiles, then the tail's blocks (global_blocks) */
+        const dim3 grid(a->t.ntiles + (a->global_blocks > 0 ? a->global_blocks : 0));
         if (a->fixed && dw)
-            hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true, ZK_WIN_DEFAULT>), dim3(a->t.ntiles), dim3(256),
-                               lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true, ZK_WIN_DEFAULT>), grid, dim3(256), lds, s, *a);
         else if (a->fixed)
-            hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<ZK_PUSH_OCC, 1, true>), grid, dim3(256), lds, s, *a);
         else if (a->waves == 8)
-            hipLaunchKernelGGL((push_tile_kernel<8, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<8, 1, false>), grid, dim3(256), lds, s, *a);
         else if (a->waves == 2)
-            hipLaunchKernelGGL((push_tile_kernel<1, 2, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<1, 2, false>), grid, dim3(256), lds, s, *a);
         else
-            hipLaunchKernelGGL((push_tile_kernel<1, 1, false>), dim3(a->t.ntiles), dim3(256), lds, s, *a);
+            hipLaunchKernelGGL((push_tile_kernel<1, 1, false>), grid, dim3(256), lds, s, *a);
+    } else if (a->global_blocks > 0) {
+        hipLaunchKernelGGL(push_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
     }
-    if (a->global_blocks > 0) hipLaunchKernelGGL(push_global_kernel, dim3(a->global_blocks), dim3(256), 0, s, *a);
     ZK_LAUNCH_CHECK();
 }
 
