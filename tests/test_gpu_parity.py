@@ -342,6 +342,26 @@ def test_gpu_smoothing_matches_filter_model(gpu):
         b.close()
 
 
+def test_gpu_one_x_one_y_pass_matches_filter_model(gpu):
+    """One binomial pass along x and one along y (the 2048^2 benchmark's
+    filter) run as one launch (smooth_xy_kernel): the filtered J is the FP32
+    model of the reference's two passes bit for bit, with the y pass reading
+    the x pass's values on interior rows and the unfiltered guard rows."""
+    base = weibel(64, 48, 2)
+    sm = weibel(64, 48, 2, smooth={"xtype": "binomial", "ytype": "binomial", "xlevel": 1, "ylevel": 1})
+    a, b = gpu_run(gpu, base), gpu_run(gpu, sm)
+    try:
+        a.step(1)
+        b.step(1)
+        f32 = np.float32
+        exp32 = filter_model(a.state()["J"], [(0.25, 0.5)], [(0.25, 0.5)], dtype=f32)
+        Js = b.state()["J"].astype(f32)
+        assert Js[1:49, 1:65].tobytes() == exp32[1:49, 1:65].tobytes()
+    finally:
+        a.close()
+        b.close()
+
+
 def test_external_and_custom_fields(gpu, oracle):
     cfg = weibel(32, 32, 2)
     cfg["ext_fld"] = {"E_0": [0.01, 0.0, 0.0],

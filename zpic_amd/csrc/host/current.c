@@ -181,6 +181,16 @@ void current_update(t_current *c)
     int axis[maxp];
     float sa[maxp], sb[maxp];
     const int np = zpic_current_passes(c, axis, sa, sb, maxp);
+    if (ZPIC_SMOOTH_X_FUSE && np == 2 && axis[0] == 0 && axis[1] == 1 && !d->open_y) {
+        /* one x and one y pass: one launch */
+        ZCHECK(zk_smooth_xy(d->J[d->jcur], d->J[1 - d->jcur], c->nx[0], c->nx[1], c->nrow, sa[0], sb[0], sa[1], sb[1],
+                            !c->moving_window, zpic_stream()));
+        d->jcur = 1 - d->jcur;
+        zpic_ktimer_end(ZPIC_T_CURRENT);
+        c->iter++;
+        d->host_valid = 0;
+        return;
+    }
     for (int k = 0; k < np;) {
         /* consecutive x passes run in one launch */
         int run = 1;

@@ -29,6 +29,12 @@ __global__ void smooth_kernel(const float *__restrict__ in, float *__restrict__ 
     smooth_node(in, out, nx, ny, nrow, axis, sa, sb, periodic_x, blockIdx.x * blockDim.x + threadIdx.x, open_y);
 }
 
+__global__ void smooth_xy_kernel(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nrow,
+                                 float sax, float sbx, float say, float sby, int periodic_x)
+{
+    smooth_xy_node(in, out, nx, ny, nrow, sax, sbx, say, sby, periodic_x, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 /* Consecutive x passes in one launch: a block stages SX_R buffer rows of
  * SX_W output columns plus npass columns on each side in LDS and applies
  * every pass there (the valid region shrinks by one column per pass), with
@@ -143,5 +149,14 @@ extern "C" int zk_smooth(const float *in, float *out, int nx, int ny, int nrow, 
     const int nodes = (nx + 3) * (ny + 3);
     hipLaunchKernelGGL(smooth_kernel, dim3((nodes + 255) / 256), dim3(256), 0, s, in, out, nx, ny, nrow, axis, sa, sb,
                        periodic_x, open_y);
+    ZK_LAUNCH_CHECK();
+}
+
+extern "C" int zk_smooth_xy(const float *in, float *out, int nx, int ny, int nrow, float sax, float sbx, float say,
+                            float sby, int periodic_x, zk_stream_t s)
+{
+    const int nodes = (nx + 3) * (ny + 3);
+    hipLaunchKernelGGL(smooth_xy_kernel, dim3((nodes + 255) / 256), dim3(256), 0, s, in, out, nx, ny, nrow, sax, sbx,
+                       say, sby, periodic_x);
     ZK_LAUNCH_CHECK();
 }

@@ -107,6 +107,52 @@ __device__ inline void smooth_node(const float *__restrict__ in, float *__restri
 #undef AT
 }
 
+/* One x pass and then one y pass (kernel_x, kernel_y, em2d/current.c:316-413)
+ * in one launch, periodic y: node (i, j) of the y pass reads the x pass's
+ * values at (i, c - 1 .. c + 1), c the image row of j, and each of those is
+ * formed here from `in` with smooth_node's x rules (guard rows copied,
+ * guard columns from their images unless the window moves) — the same
+ * operations in the same order as the two launches, without writing and
+ * re-reading the intermediate grid. */
+__device__ inline void x_pass_value(const float *__restrict__ in, int nx, int ny, int nrow, float sa, float sb,
+                                    int periodic_x, int i, int r, float *o)
+{
+#define AT(a, b) (in + 3 * ((size_t)((a) + 1) + (size_t)((b) + 1) * nrow))
+    if (r >= 0 && r < ny && ((i >= 0 && i < nx) || periodic_x)) {
+        const int c = (i >= 0 && i < nx) ? i : zk_wrap(i, nx);
+        stencil(sa, sb, AT(c - 1, r), AT(c, r), AT(c + 1, r), o);
+    } else {
+        const float *s = AT(i, r);
+        o[0] = s[0];
+        o[1] = s[1];
+        o[2] = s[2];
+    }
+#undef AT
+}
+
+__device__ inline void smooth_xy_node(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
+                                      int nrow, float sax, float sbx, float say, float sby, int periodic_x,
+                                      const int node)
+{
+    const int bw = nx + 3, bh = ny + 3;
+    if (node >= bw * bh) return;
+    const int i = node % bw - 1, j = node / bw - 1;
+    const int c = (j >= 0 && j < ny) ? j : zk_wrap(j, ny);
+    float *o = out + 3 * (size_t)node;
+    float m[3];
+    x_pass_value(in, nx, ny, nrow, sax, sbx, periodic_x, i, c, m);
+    if (i >= 0 && i < nx) {
+        float l[3], u[3];
+        x_pass_value(in, nx, ny, nrow, sax, sbx, periodic_x, i, c - 1, l);
+        x_pass_value(in, nx, ny, nrow, sax, sbx, periodic_x, i, c + 1, u);
+        stencil(say, sby, l, m, u, o);
+    } else {
+        o[0] = m[0];
+        o[1] = m[1];
+        o[2] = m[2];
+    }
+}
+
 } // namespace zk
 
 #endif

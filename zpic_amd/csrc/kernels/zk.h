@@ -202,6 +202,10 @@ int zk_smooth(const float *in, float *out, int nx, int ny, int nrow, int axis,
                      float sa, float sb, int periodic_x, int open_y, zk_stream_t s);
 /* npass (<= ZK_MAX_PASSES) consecutive x passes in one launch, bit-identical
  * to npass zk_smooth launches along x */
+/* one x pass, then one y pass with periodic y, in one launch (the same
+ * values as two zk_smooth launches) */
+int zk_smooth_xy(const float *in, float *out, int nx, int ny, int nrow, float sax, float sbx, float say, float sby,
+                 int periodic_x, zk_stream_t s);
 int zk_smooth_x_passes(const float *in, float *out, int nx, int ny, int nrow, int npass, const float *sa,
                        const float *sb, int periodic_x, zk_stream_t s);
 
