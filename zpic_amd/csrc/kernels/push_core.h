@@ -556,7 +556,14 @@ __device__ inline void sort_write(const zk_push_args &a, int d, int i, const Par
 {
     if (d >= 0) {
         store_part(a.sdst, d, c);
-        if (a.p.id) a.sdst.id[d] = a.p.id[i];
+        if (a.p.id) {
+            /* the id addresses are formed here from the slot numbers: left
+             * to itself the compiler keeps 64-bit per-lane address terms
+             * alive across the push loop, which spills registers */
+            int src = i, dst = d;
+            asm volatile("" : "+v"(src), "+v"(dst));
+            a.sdst.id[dst] = a.p.id[src];
+        }
     }
 }
 
