@@ -31,15 +31,17 @@
 
 using namespace zk;
 
-/* waves per SIMD the fused push is compiled for */
+/* waves per SIMD of the single-species tile push: 5 fits 96 VGPRs, with a
+ * few values spilled outside the loop (LWFA 4000 x 256: 0.2537 -> 0.2505
+ * ms per step against 4 waves) */
 #ifndef ZK_PUSH_OCC
-#define ZK_PUSH_OCC 4
+#define ZK_PUSH_OCC 5
 #endif
-/* waves per SIMD of the fused multi-species push: 5 fits 96 VGPRs, for
- * which the compiler spills a few loop invariants (52 bytes per lane,
- * stored before the loop, two 8-byte reloads per three particles); the
- * extra wave hides more latency (2048^2 Weibel: 1.750 -> 1.727 ms per
- * launch; 4 waves use 116 VGPRs and no scratch) */
+/* waves per SIMD of the fused multi-species push: 5 fits 96 VGPRs, with
+ * 16 bytes per lane spilled before the loop and reloaded after it; the
+ * extra wave hides more latency (2048^2 Weibel: 5 waves 1.727 ms per
+ * launch against 1.750 with 4, when the 5-wave build still reloaded two
+ * spilled address terms in the loop; 1.697 ms without them) */
 #ifndef ZK_SPECIES_OCC
 #define ZK_SPECIES_OCC 5
 #endif
